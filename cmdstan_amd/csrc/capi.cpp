@@ -1,0 +1,801 @@
+// C-ABI implementation: data contexts, model registry and the batched NUTS
+// driver (warmup phase, then sampling phase, every chain of every dataset in
+// one batch on one GPU).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cmdstan_amd.h"
+#include "json.h"
+#include "launch.h"
+#include "nuts_types.h"
+
+using namespace nutsgpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPC(x)                                                                                         \
+  do {                                                                                                  \
+    hipError_t e_ = (x);                                                                                \
+    if (e_ != hipSuccess)                                                                               \
+      throw HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" +      \
+                     std::to_string(__LINE__));                                                         \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() {}
+  explicit DevBuf(size_t n_) { alloc(n_); }
+  void alloc(size_t n_) {
+    free();
+    n = n_;
+    if (n) HIPC(hipMalloc(&p, n * sizeof(T)));
+  }
+  void upload(const T* h, size_t cnt, hipStream_t st = 0) {
+    if (cnt) HIPC(hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void upload(const std::vector<T>& h, hipStream_t st = 0) {
+    if (h.size() > n) alloc(h.size());
+    upload(h.data(), h.size(), st);
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// ------------------------------------------------------------------ data
+struct Field {
+  std::vector<int64_t> dims;
+  std::vector<double> v;
+  bool is_int = false;
+};
+
+}  // namespace
+
+struct nuts_data {
+  std::map<std::string, Field> f;
+  const Field& get(const std::string& n) const {
+    auto it = f.find(n);
+    if (it == f.end()) throw std::invalid_argument("variable does not exist; variable name = " + n);
+    return it->second;
+  }
+  bool has(const std::string& n) const { return f.count(n) != 0; }
+  double scalar(const std::string& n) const {
+    const Field& x = get(n);
+    if (x.v.size() != 1) throw std::invalid_argument("variable " + n + " must be a scalar");
+    return x.v[0];
+  }
+  double scalar_or(const std::string& n, double d) const { return has(n) ? scalar(n) : d; }
+  int integer(const std::string& n) const {
+    const double v = scalar(n);
+    if (v != std::floor(v)) throw std::invalid_argument("variable " + n + " must be an integer");
+    return (int)v;
+  }
+  const std::vector<double>& vec(const std::string& n, int64_t expect) const {
+    const Field& x = get(n);
+    if (expect >= 0 && (int64_t)x.v.size() != expect)
+      throw std::invalid_argument("variable " + n + ": size mismatch; expecting " + std::to_string(expect) +
+                                  " values, found " + std::to_string(x.v.size()));
+    return x.v;
+  }
+};
+
+namespace {
+
+// -------------------------------------------------------------- models
+struct Dataset {
+  int D = 0, nout = 0;
+  std::vector<std::string> names, unames;
+  std::vector<double> block;
+};
+
+struct ModelImpl {
+  std::string family;
+  int kind = 0;
+  std::vector<Dataset> ds;
+  DevBuf<double> d_data;
+  std::vector<int64_t> ds_off;
+  // logistic regression batched path (single dataset)
+  bool glm = false;
+  int glmN = 0, glmD = 0;
+  double glm_s2inv = 0;
+  const double* d_X = nullptr;
+  const double* d_y = nullptr;
+};
+
+std::vector<std::string> idx_names(const std::string& base, int n) {
+  std::vector<std::string> r;
+  for (int i = 0; i < n; ++i) r.push_back(base + "." + std::to_string(i + 1));
+  return r;
+}
+
+void check_binary(const std::vector<double>& y, const std::string& n) {
+  for (double v : y)
+    if (v != 0 && v != 1) throw std::domain_error(n + " is " + std::to_string(v) + ", but must be in the interval [0, 1]");
+}
+
+Dataset make_dataset(const std::string& fam, const nuts_data& d, int& kind) {
+  Dataset s;
+  if (fam == "bernoulli") {
+    kind = MK_BERNOULLI;
+    const int N = d.integer("N");
+    if (N < 0) throw std::domain_error("N is " + std::to_string(N) + ", but must be greater than or equal to 0");
+    const auto& y = d.vec("y", N);
+    check_binary(y, "y");
+    double sy = 0;
+    for (double v : y) sy += v;
+    s.D = 1; s.nout = 1;
+    s.names = {"theta"}; s.unames = {"theta"};
+    s.block = {(double)N, sy};
+  } else if (fam == "gauss") {
+    kind = MK_GAUSS;
+    const int D = d.integer("D");
+    if (D < 1) throw std::domain_error("D must be positive");
+    std::vector<double> mu = d.has("mu") ? d.vec("mu", D) : std::vector<double>(D, 0.0);
+    std::vector<double> P;
+    if (d.has("P")) P = d.vec("P", (int64_t)D * D);
+    else { P.assign((size_t)D * D, 0.0); for (int i = 0; i < D; ++i) P[(size_t)i * D + i] = 1.0; }
+    s.D = D; s.nout = D;
+    s.names = idx_names("x", D); s.unames = s.names;
+    s.block.push_back(D);
+    s.block.insert(s.block.end(), mu.begin(), mu.end());
+    s.block.insert(s.block.end(), P.begin(), P.end());
+  } else if (fam == "eight_schools_cp" || fam == "hier_normal_nc") {
+    const bool nc = fam == "hier_normal_nc";
+    kind = nc ? MK_HIER_NORMAL_NC : MK_EIGHT_SCHOOLS;
+    const int J = d.integer("J");
+    if (J < 1) throw std::domain_error("J must be positive");
+    const auto& y = d.vec("y", J);
+    const auto& sg = d.vec("sigma", J);
+    for (double v : sg) if (!(v > 0)) throw std::domain_error("sigma must be positive");
+    const double mu_s = d.scalar_or("mu_scale", nc ? 5.0 : 0.0);
+    const double tau_s = d.scalar_or("tau_scale", nc ? 5.0 : 0.0);
+    s.D = J + 2;
+    s.nout = nc ? 2 + 2 * J : 2 + J;
+    s.names = {"mu", "tau"};
+    s.unames = {"mu", "tau"};
+    auto th = idx_names(nc ? "eta" : "theta", J);
+    s.names.insert(s.names.end(), th.begin(), th.end());
+    s.unames.insert(s.unames.end(), th.begin(), th.end());
+    if (nc) { auto t2 = idx_names("theta", J); s.names.insert(s.names.end(), t2.begin(), t2.end()); }
+    s.block = {(double)J, mu_s, tau_s};
+    s.block.insert(s.block.end(), y.begin(), y.end());
+    s.block.insert(s.block.end(), sg.begin(), sg.end());
+  } else if (fam == "hier_logistic_cp") {
+    kind = MK_HIER_LOGISTIC;
+    const int N = d.integer("N"), K = d.integer("K"), G = d.integer("G");
+    if (K < 1 || K > 16) throw std::domain_error("K must be in [1, 16]");
+    if (G < 1) throw std::domain_error("G must be positive");
+    const auto& X = d.vec("X", (int64_t)N * K);
+    const auto& y = d.vec("y", N);
+    const auto& gg = d.vec("group", N);
+    check_binary(y, "y");
+    std::vector<int> grp(N);
+    for (int n = 0; n < N; ++n) {
+      grp[n] = (int)gg[n] - 1;
+      if (grp[n] < 0 || grp[n] >= G) throw std::domain_error("group out of range");
+    }
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return grp[a] < grp[b]; });
+    std::vector<double> gstart(G + 1, 0.0);
+    for (int n = 0; n < N; ++n) gstart[grp[n] + 1] += 1;
+    for (int g = 0; g < G; ++g) gstart[g + 1] += gstart[g];
+    s.D = 2 * K + G * K;
+    s.nout = s.D;
+    s.names = idx_names("mu", K);
+    auto t = idx_names("tau", K);
+    s.names.insert(s.names.end(), t.begin(), t.end());
+    s.unames = s.names;
+    for (int k = 0; k < K; ++k)
+      for (int g = 0; g < G; ++g) s.names.push_back("beta." + std::to_string(g + 1) + "." + std::to_string(k + 1));
+    for (int g = 0; g < G; ++g)
+      for (int k = 0; k < K; ++k) s.unames.push_back("beta." + std::to_string(g + 1) + "." + std::to_string(k + 1));
+    s.block = {(double)N, (double)K, (double)G, d.scalar_or("mu_scale", 5.0)};
+    s.block.insert(s.block.end(), gstart.begin(), gstart.end());
+    for (int n : order) for (int k = 0; k < K; ++k) s.block.push_back(X[(size_t)n * K + k]);
+    for (int n : order) s.block.push_back(y[n]);
+  } else if (fam == "logistic_regression") {
+    kind = MK_LOGISTIC;
+    const int N = d.integer("N"), D = d.integer("D");
+    if (N < 0 || D < 1) throw std::domain_error("bad N or D");
+    const auto& X = d.vec("X", (int64_t)N * D);
+    const auto& y = d.vec("y", N);
+    check_binary(y, "y");
+    const double sc = d.scalar_or("prior_scale", 2.5);
+    s.D = D; s.nout = D;
+    s.names = idx_names("beta", D); s.unames = s.names;
+    s.block.reserve(3 + (size_t)N * D + N);
+    s.block = {(double)N, (double)D, 1.0 / (sc * sc)};
+    s.block.insert(s.block.end(), X.begin(), X.end());
+    s.block.insert(s.block.end(), y.begin(), y.end());
+  } else {
+    throw std::invalid_argument("unknown model family: " + fam);
+  }
+  return s;
+}
+
+// ------------------------------------------------------------- windows
+// stan::mcmc::windowed_adaptation::set_window_params + restart (see the
+// device logic in nuts_device.h); returns the initial per-chain window state.
+struct WindowInit {
+  uint32_t num_warmup = 0, init_buffer = 0, term_buffer = 0, base_window = 0;
+  uint32_t counter = 0, size = 0, next = 0;
+};
+WindowInit window_init(unsigned nw, unsigned ib, unsigned tb, unsigned bw) {
+  WindowInit w;
+  auto restart = [&]() { w.counter = 0; w.size = w.base_window; w.next = w.init_buffer + w.size - 1; };
+  restart();  // constructor state
+  if (nw < 20) return w;
+  if (ib + bw + tb > nw) {
+    w.num_warmup = nw;
+    w.init_buffer = (uint32_t)(0.15 * nw);
+    w.term_buffer = (uint32_t)(0.1 * nw);
+    w.base_window = nw - (w.init_buffer + w.term_buffer);
+    return w;  // no restart: the schedule keeps its constructor state
+  }
+  w.num_warmup = nw; w.init_buffer = ib; w.term_buffer = tb; w.base_window = bw;
+  restart();
+  return w;
+}
+
+// ---------------------------------------------------------------- runs
+struct RunSpec {
+  nuts_config cfg;
+  const double* init = nullptr;        // [C][D]
+  const double* inv_metric = nullptr;  // [C][D]
+  const double* inj_u = nullptr;
+  int64_t n_u = 0;
+  const double* inj_z = nullptr;
+  int64_t n_z = 0;
+  int first_dataset = 0, num_datasets = -1;
+  bool constrain = true;
+};
+
+struct RunResult {
+  std::vector<double> draws;  // constrained (or unconstrained if !constrain) rows, chains back to back
+  std::vector<int64_t> draw_off;
+  std::vector<int> cols;
+  std::vector<ChainState> st;
+  std::vector<double> minv;  // compact [C][D]
+  double warm_s = 0, samp_s = 0;
+  int64_t leaf_steps = 0;
+  double grad_s = 0;
+  int64_t grad_timed = 0;
+  int rows = 0;
+};
+
+int output_rows(const nuts_config& c) {
+  int r = 0;
+  if (c.save_warmup) r += (c.num_warmup + c.thin - 1) / c.thin;
+  r += (c.num_samples + c.thin - 1) / c.thin;
+  return r;
+}
+
+void validate(const nuts_config& c) {
+  if (c.num_samples < 0) throw std::invalid_argument("num_samples must be >= 0");
+  if (c.num_warmup < 0) throw std::invalid_argument("num_warmup must be >= 0");
+  if (c.thin < 1) throw std::invalid_argument("thin must be > 0");
+  if (c.adapt_engaged && c.num_warmup == 0)
+    throw std::invalid_argument("The number of warmup samples (num_warmup) must be greater than zero if adaptation "
+                                "is enabled.");
+  if (!(c.delta >= 0.1 && c.delta < 1.0)) throw std::invalid_argument("delta must be in [0.1, 1)");
+  if (!(c.gamma > 0)) throw std::invalid_argument("gamma must be positive");
+  if (!(c.kappa > 0)) throw std::invalid_argument("kappa must be positive");
+  if (!(c.t0 > 0)) throw std::invalid_argument("t0 must be positive");
+  if (!(c.stepsize > 0)) throw std::invalid_argument("stepsize must be positive");
+  if (!(c.stepsize_jitter >= 0 && c.stepsize_jitter <= 1)) throw std::invalid_argument("stepsize_jitter must be in [0, 1]");
+  if (c.max_depth < 1) throw std::invalid_argument("max_depth must be > 0");
+  if (c.max_depth > NUTS_MAX_DEPTH)
+    throw std::invalid_argument("max_depth above " + std::to_string(NUTS_MAX_DEPTH) + " is not supported");
+  if (c.metric < 0 || c.metric > 2) throw std::invalid_argument("metric must be unit_e, diag_e or dense_e");
+  if (c.metric == NUTS_METRIC_DENSE_E) throw std::invalid_argument("dense_e is not available in this build");
+  if (c.num_chains < 1) throw std::invalid_argument("num_chains must be >= 1");
+  if (c.init_radius < 0) throw std::invalid_argument("init radius must be >= 0");
+}
+
+__global__ void resume_kernel(ChainState* st, int C) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C && st[c].phase == PH_PAUSED) st[c].phase = PH_TRANS_BEGIN;
+}
+
+RunResult run_batch(ModelImpl& M, const RunSpec& R) {
+  const nuts_config& cfg = R.cfg;
+  validate(cfg);
+  const int nds = R.num_datasets < 0 ? (int)M.ds.size() - R.first_dataset : R.num_datasets;
+  const int C = nds * cfg.num_chains;
+  const int rows = output_rows(cfg);
+  std::vector<int> D(C), Dp(C), nout(C);
+  std::vector<int64_t> off(C), moff(C), doff(C), dsoff(C), ooff(C);
+  std::vector<uint32_t> uid(C);
+  int64_t tot = 0, dtot = 0, otot = 0;
+  for (int c = 0; c < C; ++c) {
+    const int m = R.first_dataset + c / cfg.num_chains;
+    D[c] = M.ds[m].D;
+    Dp[c] = (D[c] + 15) / 16 * 16;
+    nout[c] = M.ds[m].nout;
+    off[c] = tot;
+    moff[c] = 0;
+    tot += Dp[c];
+    doff[c] = dtot;
+    dtot += (int64_t)rows * (7 + D[c]);
+    ooff[c] = otot;
+    otot += (int64_t)rows * (7 + nout[c]);
+    dsoff[c] = M.ds_off[m];
+    uid[c] = (uint32_t)(cfg.id + c);
+  }
+  const int maxd = cfg.max_depth;
+  const int nslots = maxd + 2;
+  const int NV = PV_FIXED + 3 * maxd + 2 * nslots;
+
+  hipStream_t st;
+  HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{st};
+
+  DevBuf<int> dD(C), dDp(C), dneed(C), dstop(1);
+  DevBuf<int64_t> doffb(C), dmoff(C), ddoff(C), ddsoff(C), dooff(C);
+  DevBuf<uint32_t> duid(C);
+  DevBuf<double> Q(tot), G(tot), V(C), pool((size_t)tot * NV), minv(tot), draws(std::max<int64_t>(dtot, 1));
+  DevBuf<ChainState> dst(C);
+  DevBuf<double> uinit, du, dz;
+  dD.upload(D, st); dDp.upload(Dp, st); doffb.upload(off, st); dmoff.upload(moff, st); ddoff.upload(doff, st);
+  ddsoff.upload(dsoff, st); dooff.upload(ooff, st); duid.upload(uid, st);
+  HIPC(hipMemsetAsync(Q.p, 0, tot * sizeof(double), st));
+  HIPC(hipMemsetAsync(G.p, 0, tot * sizeof(double), st));
+  HIPC(hipMemsetAsync(V.p, 0, C * sizeof(double), st));
+  HIPC(hipMemsetAsync(pool.p, 0, (size_t)tot * NV * sizeof(double), st));
+  HIPC(hipMemsetAsync(dneed.p, 0, C * sizeof(int), st));
+  HIPC(hipMemsetAsync(dstop.p, 0, sizeof(int), st));
+  {
+    std::vector<double> hm(tot, 1.0);
+    if (R.inv_metric && cfg.metric != NUTS_METRIC_UNIT_E) {
+      for (int c = 0; c < C; ++c)
+        for (int i = 0; i < D[c]; ++i) {
+          const double v = R.inv_metric[(size_t)c * D[0] + i];
+          if (!(v > 0) || !std::isfinite(v)) throw std::invalid_argument("inverse metric must be positive and finite");
+          hm[off[c] + i] = v;
+        }
+    }
+    minv.upload(hm, st);
+  }
+  if (R.init) {
+    std::vector<double> hi(tot, 0.0);
+    for (int c = 0; c < C; ++c)
+      for (int i = 0; i < D[c]; ++i) hi[off[c] + i] = R.init[(size_t)c * D[0] + i];
+    uinit.upload(hi, st);
+  }
+  if (R.inj_u) { du.alloc((size_t)C * std::max<int64_t>(R.n_u, 1)); du.upload(R.inj_u, (size_t)C * R.n_u, st); }
+  if (R.inj_z) { dz.alloc((size_t)C * std::max<int64_t>(R.n_z, 1)); dz.upload(R.inj_z, (size_t)C * R.n_z, st); }
+
+  const WindowInit w = window_init(cfg.num_warmup, cfg.init_buffer, cfg.term_buffer, cfg.window);
+  std::vector<ChainState> hs(C);
+  for (int c = 0; c < C; ++c) {
+    ChainState& s = hs[c];
+    std::memset(&s, 0, sizeof(s));
+    s.phase = PH_INIT_TRY;
+    s.sample_slot = 0;
+    s.free_mask = ((nslots >= 32) ? 0xFFFFFFFFu : ((1u << nslots) - 1u)) & ~1u;
+    s.nom_eps = cfg.stepsize;
+    s.eps = cfg.stepsize;
+    s.adapt_flag = cfg.adapt_engaged ? 1 : 0;
+    s.sa_mu = std::log(10 * cfg.stepsize);
+    s.w_counter = w.counter; s.w_size = w.size; s.w_next = w.next;
+    for (int k = 0; k < NUTS_MAX_DEPTH; ++k) s.L_prop[k] = -1;
+  }
+  dst.upload(hs, st);
+
+  NutsParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.num_warmup = cfg.num_warmup; P.num_samples = cfg.num_samples; P.thin = cfg.thin;
+  P.save_warmup = cfg.save_warmup; P.adapt = cfg.adapt_engaged; P.max_depth = cfg.max_depth;
+  P.max_init_tries = (R.init || cfg.init_radius == 0) ? 1 : 100;
+  P.iter_target = cfg.num_warmup;
+  P.delta = cfg.delta; P.gamma = cfg.gamma; P.kappa = cfg.kappa; P.t0 = cfg.t0;
+  P.jitter = cfg.stepsize_jitter; P.stepsize = cfg.stepsize; P.init_radius = cfg.init_radius;
+  P.max_deltaH = 1000;
+  P.w_num_warmup = w.num_warmup; P.w_init_buffer = w.init_buffer; P.w_term_buffer = w.term_buffer;
+  P.w_base_window = w.base_window; P.w_counter0 = w.counter; P.w_size0 = w.size; P.w_next0 = w.next;
+  P.metric_adapt = 1;
+  P.seed = cfg.seed;
+  P.nslots = nslots; P.maxd = maxd; P.NV = NV;
+  P.fused_budget = 256;
+
+  NutsBuffers B;
+  std::memset(&B, 0, sizeof(B));
+  B.C = C; B.D = dD.p; B.Dp = dDp.p; B.off = doffb.p; B.moff = dmoff.p; B.uid = duid.p;
+  B.Q = Q.p; B.G = G.p; B.V = V.p; B.pool = pool.p; B.st = dst.p; B.minv = minv.p;
+  B.draws = draws.p; B.draw_off = ddoff.p; B.user_init = R.init ? uinit.p : nullptr;
+  B.n_stopped = dstop.p; B.need = dneed.p;
+  B.inj_u = R.inj_u ? du.p : nullptr; B.inj_z = R.inj_z ? dz.p : nullptr;
+  B.n_inj_u = R.n_u; B.n_inj_z = R.n_z;
+
+  ModelDataDev md{M.d_data.p, ddsoff.p, 1.0};
+
+  bool fused = cfg.fused == 1;
+  const bool glm_ok = M.glm && nds == 1;
+  if (cfg.fused < 0) fused = !glm_ok;
+  if (!fused && M.kind == MK_LOGISTIC && !glm_ok) fused = false;  // per-chain gradient kernel in lockstep
+
+  DevBuf<double> work;
+  GlmArgs ga;
+  const int rows_per_chunk = 1024;
+  if (!fused && glm_ok) {
+    work.alloc(glm_workspace_doubles(M.glmN, C, M.glmD, rows_per_chunk));
+    ga.X = M.d_X; ga.y = M.d_y; ga.N = M.glmN; ga.D = M.glmD; ga.C = C; ga.rows_per_chunk = rows_per_chunk;
+    ga.s2inv = M.glm_s2inv; ga.Q = Q.p; ga.ldq = Dp[0]; ga.G = G.p; ga.V = V.p; ga.work = work.p;
+  }
+
+  RunResult res;
+  res.rows = rows;
+  int stopped = 0;
+  hipEvent_t ev0, ev1;
+  HIPC(hipEventCreate(&ev0));
+  HIPC(hipEventCreate(&ev1));
+  struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{ev0, ev1};
+  const char* lim = std::getenv("CMDSTAN_AMD_MAX_SECONDS");
+  const double max_seconds = lim ? std::atof(lim) : 3600.0;
+  const auto t_begin = std::chrono::steady_clock::now();
+  auto run_phase = [&](bool time_grad) {
+    const int check_every = fused ? 1 : 16;
+    for (;;) {
+      if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() > max_seconds)
+        throw std::runtime_error("sampler did not finish within CMDSTAN_AMD_MAX_SECONDS");
+      bool timed = false;
+      for (int k = 0; k < check_every; ++k) {
+        if (fused) {
+          launch_advance_fused(M.kind, cfg.metric, P, B, md, st);
+        } else {
+          launch_advance_external(cfg.metric, P, B, st);
+          const bool t = time_grad && k == check_every / 2;
+          if (t) HIPC(hipEventRecord(ev0, st));
+          if (glm_ok) { if (glm_grad_f64(ga, st)) throw std::runtime_error("GLM kernel: unsupported D"); }
+          else launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
+          if (t) { HIPC(hipEventRecord(ev1, st)); timed = true; }
+          res.leaf_steps++;
+        }
+      }
+      HIPC(hipGetLastError());
+      HIPC(hipMemcpyAsync(&stopped, dstop.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      if (timed) {
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        res.grad_s += ms * 1e-3;
+        res.grad_timed++;
+      }
+      if (stopped >= C) break;
+    }
+  };
+
+  HIPC(hipStreamSynchronize(st));
+  auto t0 = std::chrono::steady_clock::now();
+  run_phase(false);
+  auto t1 = std::chrono::steady_clock::now();
+  // resume paused chains for the sampling phase
+  {
+    std::vector<ChainState> cur(C);
+    HIPC(hipMemcpyAsync(cur.data(), dst.p, C * sizeof(ChainState), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    int done = 0;
+    for (auto& s : cur) if (s.phase == PH_DONE || s.phase == PH_FAILED) ++done;
+    HIPC(hipMemcpyAsync(dstop.p, &done, sizeof(int), hipMemcpyHostToDevice, st));
+  }
+  hipLaunchKernelGGL(resume_kernel, dim3((C + 255) / 256), dim3(256), 0, st, dst.p, C);
+  P.iter_target = cfg.num_warmup + cfg.num_samples;
+  HIPC(hipStreamSynchronize(st));
+  auto t2 = std::chrono::steady_clock::now();
+  run_phase(true);
+  auto t3 = std::chrono::steady_clock::now();
+  res.warm_s = std::chrono::duration<double>(t1 - t0).count();
+  res.samp_s = std::chrono::duration<double>(t3 - t2).count();
+
+  res.st.resize(C);
+  HIPC(hipMemcpyAsync(res.st.data(), dst.p, C * sizeof(ChainState), hipMemcpyDeviceToHost, st));
+  {
+    std::vector<double> hm(tot);
+    HIPC(hipMemcpyAsync(hm.data(), minv.p, tot * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    for (int c = 0; c < C; ++c) for (int i = 0; i < D[c]; ++i) res.minv.push_back(hm[off[c] + i]);
+  }
+  if (R.constrain) {
+    DevBuf<double> out(std::max<int64_t>(otot, 1));
+    DevBuf<int> dn(C);
+    dn.upload(nout, st);
+    launch_write_array(M.kind, C, rows, dD.p, ddoff.p, dooff.p, dn.p, draws.p, out.p, md, st);
+    res.draws.resize(otot);
+    HIPC(hipMemcpyAsync(res.draws.data(), out.p, otot * sizeof(double), hipMemcpyDeviceToHost, st));
+    res.draw_off = ooff;
+    for (int c = 0; c < C; ++c) res.cols.push_back(7 + nout[c]);
+  } else {
+    res.draws.resize(dtot);
+    HIPC(hipMemcpyAsync(res.draws.data(), draws.p, dtot * sizeof(double), hipMemcpyDeviceToHost, st));
+    res.draw_off = doff;
+    for (int c = 0; c < C; ++c) res.cols.push_back(7 + D[c]);
+  }
+  HIPC(hipStreamSynchronize(st));
+  HIPC(hipGetLastError());
+  return res;
+}
+
+const char* failure_text(int f) {
+  switch (f) {
+    case FAIL_INIT: return "Initialization failed.";
+    case FAIL_STEPSIZE_IMPROPER: return "Posterior is improper. Please check your model.";
+    case FAIL_STEPSIZE_ZERO: return "No acceptably small step size could be found. Perhaps the posterior is not continuous?";
+    case FAIL_METRIC_OVERFLOW: return "Numerical overflow in metric adaptation.";
+    default: return "";
+  }
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NUTS_NOT_OK;
+  }
+}
+
+}  // namespace
+
+struct nuts_model {
+  ModelImpl impl;
+};
+
+extern "C" {
+
+const char* nuts_last_error(void) { return g_err.c_str(); }
+const char* nuts_version(void) { return "cmdstan_amd 0.1 (gfx950)"; }
+
+nuts_data* nuts_data_create(void) { return new nuts_data(); }
+void nuts_data_destroy(nuts_data* d) { delete d; }
+
+int nuts_data_add(nuts_data* d, const char* name, int ndim, const int64_t* dims, const double* values, int is_int) {
+  return guarded([&]() {
+    Field f;
+    int64_t n = 1;
+    for (int i = 0; i < ndim; ++i) { f.dims.push_back(dims[i]); n *= dims[i]; }
+    f.v.assign(values, values + n);
+    f.is_int = is_int != 0;
+    d->f[name] = std::move(f);
+    return NUTS_OK;
+  });
+}
+
+int nuts_data_parse_json(nuts_data* d, const char* text) {
+  return guarded([&]() {
+    std::string t(text);
+    JsonParser p(t);
+    JVal v = p.parse();
+    if (v.kind != JVal::OBJ) throw std::invalid_argument("JSON data must be an object");
+    for (auto& kv : v.obj) {
+      Field f;
+      bool all_int = true;
+      json_flatten(kv.second, f.dims, f.v, all_int, kv.first);
+      f.is_int = all_int;
+      d->f[kv.first] = std::move(f);
+    }
+    return NUTS_OK;
+  });
+}
+
+int nuts_data_get(const nuts_data* d, const char* name, int* ndim, int64_t* dims, const double** values) {
+  return guarded([&]() {
+    const Field& f = d->get(name);
+    *ndim = (int)f.dims.size();
+    for (size_t i = 0; i < f.dims.size() && i < 8; ++i) dims[i] = f.dims[i];
+    *values = f.v.data();
+    return NUTS_OK;
+  });
+}
+
+nuts_model* nuts_model_create(const char* family, const nuts_data* const* datasets, int num_datasets) {
+  try {
+    if (num_datasets < 1) throw std::invalid_argument("num_datasets must be >= 1");
+    std::unique_ptr<nuts_model> m(new nuts_model());
+    ModelImpl& M = m->impl;
+    M.family = family;
+    std::vector<double> packed;
+    for (int i = 0; i < num_datasets; ++i) {
+      int kind = 0;
+      M.ds.push_back(make_dataset(family, *datasets[i], kind));
+      M.kind = kind;
+      M.ds_off.push_back((int64_t)packed.size());
+      packed.insert(packed.end(), M.ds.back().block.begin(), M.ds.back().block.end());
+      M.ds.back().block.clear();
+      M.ds.back().block.shrink_to_fit();
+    }
+    M.d_data.alloc(std::max<size_t>(packed.size(), 1));
+    M.d_data.upload(packed.data(), packed.size());
+    HIPC(hipDeviceSynchronize());
+    if (M.kind == MK_LOGISTIC && num_datasets == 1 && M.ds[0].D <= 128) {
+      M.glm = true;
+      M.glmN = (int)packed[0];
+      M.glmD = (int)packed[1];
+      M.glm_s2inv = packed[2];
+      M.d_X = M.d_data.p + 3;
+      M.d_y = M.d_data.p + 3 + (size_t)M.glmN * M.glmD;
+    }
+    return m.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void nuts_model_destroy(nuts_model* m) { delete m; }
+int nuts_model_num_datasets(const nuts_model* m) { return (int)m->impl.ds.size(); }
+
+int nuts_model_dims(const nuts_model* m, int dataset, int* num_unconstrained, int* num_outputs) {
+  return guarded([&]() {
+    if (dataset < 0 || dataset >= (int)m->impl.ds.size()) throw std::out_of_range("dataset index");
+    if (num_unconstrained) *num_unconstrained = m->impl.ds[dataset].D;
+    if (num_outputs) *num_outputs = m->impl.ds[dataset].nout;
+    return NUTS_OK;
+  });
+}
+
+static int copy_name(const std::string& s, char* buf, int buflen) {
+  if ((int)s.size() + 1 > buflen) throw std::length_error("name buffer too small");
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return NUTS_OK;
+}
+
+int nuts_model_param_name(const nuts_model* m, int dataset, int i, char* buf, int buflen) {
+  return guarded([&]() { return copy_name(m->impl.ds.at(dataset).names.at(i), buf, buflen); });
+}
+int nuts_model_unconstrained_name(const nuts_model* m, int dataset, int i, char* buf, int buflen) {
+  return guarded([&]() { return copy_name(m->impl.ds.at(dataset).unames.at(i), buf, buflen); });
+}
+
+int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points, int jacobian, double* lp,
+                       double* grad) {
+  return guarded([&]() {
+    ModelImpl& M = m->impl;
+    if (dataset < 0 || dataset >= (int)M.ds.size()) throw std::out_of_range("dataset index");
+    if (n_points < 1) return NUTS_OK;
+    const int D = M.ds[dataset].D;
+    const int Dp = (D + 15) / 16 * 16;
+    const int C = n_points;
+    std::vector<int> Dv(C, D);
+    std::vector<int64_t> off(C), dso(C, M.ds_off[dataset]);
+    std::vector<double> hq((size_t)C * Dp, 0.0);
+    for (int c = 0; c < C; ++c) {
+      off[c] = (int64_t)c * Dp;
+      for (int i = 0; i < D; ++i) hq[(size_t)c * Dp + i] = q[(size_t)c * D + i];
+    }
+    DevBuf<int> dD(C);
+    DevBuf<int64_t> doff(C), ddso(C);
+    DevBuf<double> Q(hq.size()), G(hq.size()), V(C);
+    dD.upload(Dv); doff.upload(off); ddso.upload(dso); Q.upload(hq);
+    ModelDataDev md{M.d_data.p, ddso.p, jacobian ? 1.0 : 0.0};
+    if (M.glm && dataset == 0) {
+      const int rpc = 1024;
+      DevBuf<double> work(glm_workspace_doubles(M.glmN, C, D, rpc));
+      GlmArgs ga{M.d_X, M.d_y, M.glmN, D, C, rpc, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
+      if (glm_grad_f64(ga, 0)) throw std::runtime_error("GLM kernel: unsupported D");
+      HIPC(hipDeviceSynchronize());
+    } else {
+      launch_model_grad(M.kind, C, dD.p, doff.p, Q.p, G.p, V.p, nullptr, md, 0);
+    }
+    HIPC(hipGetLastError());
+    std::vector<double> hg(hq.size()), hv(C);
+    HIPC(hipMemcpy(hg.data(), G.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hv.data(), V.p, C * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < C; ++c) {
+      lp[c] = -hv[c];
+      for (int i = 0; i < D; ++i) grad[(size_t)c * D + i] = -hg[(size_t)c * Dp + i];
+    }
+    return NUTS_OK;
+  });
+}
+
+void nuts_config_default(nuts_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->num_samples = 1000; c->num_warmup = 1000; c->save_warmup = 0; c->thin = 1;
+  c->adapt_engaged = 1; c->gamma = 0.05; c->delta = 0.8; c->kappa = 0.75; c->t0 = 10;
+  c->init_buffer = 75; c->term_buffer = 50; c->window = 25;
+  c->max_depth = 10; c->metric = NUTS_METRIC_DIAG_E; c->stepsize = 1; c->stepsize_jitter = 0;
+  c->num_chains = 1; c->id = 1; c->seed = 0; c->init_radius = 2; c->fused = -1;
+}
+
+int nuts_config_validate(const nuts_config* cfg) {
+  return guarded([&]() { validate(*cfg); return NUTS_OK; });
+}
+
+int nuts_output_rows(const nuts_config* cfg) { return output_rows(*cfg); }
+
+int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const double* inv_metric,
+                nuts_output* out) {
+  return guarded([&]() {
+    RunSpec R;
+    R.cfg = *cfg;
+    R.init = init;
+    R.inv_metric = inv_metric;
+    if (init || inv_metric) {
+      // per-chain arrays index with a uniform stride: require one dimension
+      for (auto& d : m->impl.ds)
+        if (d.D != m->impl.ds[0].D) throw std::invalid_argument("init/inv_metric arrays need equal dimensions");
+    }
+    RunResult r = run_batch(m->impl, R);
+    const int C = (int)r.st.size();
+    if (out) {
+      if (out->draws) std::memcpy(out->draws, r.draws.data(), r.draws.size() * sizeof(double));
+      int64_t moff = 0;
+      for (int c = 0; c < C; ++c) {
+        const ChainState& s = r.st[c];
+        if (out->stepsize) out->stepsize[c] = s.nom_eps;
+        if (out->n_leapfrog) out->n_leapfrog[c] = s.n_leapfrog_total;
+        if (out->grad_evals) out->grad_evals[c] = s.n_grad;
+        if (out->status) out->status[c] = s.phase == PH_DONE ? 0 : (s.failure ? s.failure : -1);
+      }
+      if (out->inv_metric) std::memcpy(out->inv_metric, r.minv.data(), r.minv.size() * sizeof(double));
+      (void)moff;
+      out->warmup_seconds = r.warm_s;
+      out->sampling_seconds = r.samp_s;
+      out->leaf_steps = r.leaf_steps;
+      out->grad_seconds_per_launch = r.grad_timed ? r.grad_s / r.grad_timed : 0.0;
+      out->grad_launches_timed = r.grad_timed;
+    }
+    for (int c = 0; c < C; ++c)
+      if (r.st[c].phase != PH_DONE) {
+        g_err = std::string("chain ") + std::to_string(cfg->id + c) + ": " + failure_text(r.st[c].failure);
+        return NUTS_NOT_OK;
+      }
+    return NUTS_OK;
+  });
+}
+
+int nuts_transition_replay(nuts_model* m, int dataset, int num_chains, int metric, double stepsize, double jitter,
+                           int max_depth, const double* inv_metric, const double* q0, const double* u, int64_t n_u,
+                           const double* z, int64_t n_z, double* out_q, double* info) {
+  return guarded([&]() {
+    RunSpec R;
+    nuts_config_default(&R.cfg);
+    R.cfg.num_warmup = 0; R.cfg.num_samples = 1; R.cfg.adapt_engaged = 0;
+    R.cfg.metric = metric; R.cfg.stepsize = stepsize; R.cfg.stepsize_jitter = jitter;
+    R.cfg.max_depth = max_depth; R.cfg.num_chains = num_chains; R.cfg.fused = 1;
+    R.init = q0; R.inv_metric = inv_metric;
+    R.inj_u = u; R.n_u = n_u; R.inj_z = z; R.n_z = n_z;
+    R.first_dataset = dataset; R.num_datasets = 1;
+    R.constrain = false;
+    RunResult r = run_batch(m->impl, R);
+    const int D = m->impl.ds[dataset].D;
+    for (int c = 0; c < num_chains; ++c) {
+      const ChainState& s = r.st[c];
+      if (s.phase != PH_DONE) throw std::runtime_error(std::string("replay failed: ") + failure_text(s.failure));
+      const double* row = r.draws.data() + r.draw_off[c];
+      for (int i = 0; i < D; ++i) out_q[(size_t)c * D + i] = row[7 + i];
+      double* inf = info + (size_t)c * 9;
+      inf[0] = row[3]; inf[1] = row[4]; inf[2] = row[5]; inf[3] = row[1]; inf[4] = row[6]; inf[5] = row[0];
+      inf[6] = (double)s.uctr; inf[7] = (double)s.nctr; inf[8] = row[2];
+    }
+    return NUTS_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Host-side launchers for the HIP kernels (implemented in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "nuts_types.h"
+
+namespace nutsgpu {
+
+enum ModelKind : int {
+  MK_BERNOULLI = 0,
+  MK_GAUSS,
+  MK_EIGHT_SCHOOLS,
+  MK_HIER_NORMAL_NC,
+  MK_HIER_LOGISTIC,
+  MK_LOGISTIC,
+  MK_COUNT
+};
+
+struct ModelDataDev {
+  const double* data;
+  const int64_t* off;  // [C]
+  double jac;          // 1 adds the log-Jacobian (sampling), 0 for log_prob jacobian=0
+};
+
+// Advance every chain to its next gradient request (lockstep) or through up
+// to P.fused_budget gradients computed in place by the model (fused).
+void launch_advance_external(int metric, const NutsParams& P, const NutsBuffers& B, hipStream_t st);
+void launch_advance_fused(int kind, int metric, const NutsParams& P, const NutsBuffers& B, const ModelDataDev& md,
+                          hipStream_t st);
+
+// Per-chain model gradient: V[c], G row c at Q row c for chains with need[c]
+// (need == nullptr: all chains).  Clears need[c].
+void launch_model_grad(int kind, int C, const int* D, const int64_t* off, const double* Q, double* G, double* V,
+                       int* need, const ModelDataDev& md, hipStream_t st);
+
+// Constrained outputs: in rows [C][rows][7 + D_c] -> out rows [C][rows][7 + nout_c]
+void launch_write_array(int kind, int C, int rows, const int* D, const int64_t* in_off, const int64_t* out_off,
+                        const int* nout, const double* in, double* out, const ModelDataDev& md, hipStream_t st);
+
+// Batched logistic-regression gradient over a uniform-D chain batch.
+struct GlmArgs {
+  const double* X; const double* y;
+  int N, D, C, rows_per_chunk;
+  double s2inv;
+  const double* Q; int64_t ldq;
+  double* G; double* V;
+  double* work;
+};
+size_t glm_workspace_doubles(int N, int C, int D, int rows_per_chunk);
+int glm_grad_f64(const GlmArgs& a, hipStream_t st);
+
+}  // namespace nutsgpu

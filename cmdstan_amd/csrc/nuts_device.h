@@ -1,0 +1,548 @@
+// Batched NUTS as a per-chain state machine, one wavefront per chain.
+//
+// The upstream recursion (stan::mcmc::base_nuts::transition / build_tree, not in
+// the reference tree; entered from cmdstan src/cmdstan/command.hpp:603-697) is
+// unrolled into an iteration over the leaves of each doubling.  After leaf i of
+// a depth-d subtree, the subtrees that close at that leaf are merged bottom-up:
+// level k merges while bit k of i is set, exactly the post-order in which the
+// recursion finishes its calls, so the uniforms are consumed in the same order
+// and the same three U-turn checks run per merge.  Open left siblings live in
+// per-level checkpoints (first/last momentum, rho, log weight, proposal slot).
+//
+// A chain stops at every point where it needs the potential V(q) and its
+// gradient: in the lockstep mode the gradient for all chains is computed by a
+// separate kernel (GEMM for GLMs); in the fused mode a model device function is
+// called in place and the chain keeps going for a bounded number of gradients.
+//
+// All lanes of the wave execute the same scalar control flow; vectors are
+// strided over the 64 lanes and reductions are butterfly shuffles, so every
+// lane holds identical scalar state (the staged ChainState in LDS is written
+// with identical values by all lanes).
+#pragma once
+#include "device_common.h"
+#include "nuts_types.h"
+
+namespace nutsgpu {
+
+struct ExternalGrad {
+  static constexpr bool fused = false;
+  __device__ double operator()(int, const double*, double*, int, int) const { return 0.0; }
+};
+
+template <int METRIC, class GradFn>
+struct NutsWave {
+  const NutsParams& P;
+  const NutsBuffers& B;
+  ChainState& s;
+  const GradFn& grad;
+  int c, lane, D, Dp;
+  uint32_t uid;
+  double* q;     // Z end position (dense Q row)
+  double* g;     // Z end potential gradient (dense G row)
+  double* blk;   // pool block of this chain
+  double* minv;
+
+  __device__ NutsWave(const NutsParams& P_, const NutsBuffers& B_, ChainState& s_, const GradFn& g_, int c_, int lane_)
+      : P(P_), B(B_), s(s_), grad(g_), c(c_), lane(lane_) {
+    D = B.D[c];
+    Dp = B.Dp[c];
+    uid = B.uid[c];
+    q = B.Q + B.off[c];
+    g = B.G + B.off[c];
+    blk = B.pool + B.off[c] * (int64_t)P.NV;
+    minv = B.minv + B.off[c];
+  }
+
+  __device__ double* vecp(int id) const { return blk + (int64_t)id * Dp; }
+  __device__ double* Lbeg(int k) const { return vecp(PV_FIXED + k); }
+  __device__ double* Lend(int k) const { return vecp(PV_FIXED + P.maxd + k); }
+  __device__ double* Lrho(int k) const { return vecp(PV_FIXED + 2 * P.maxd + k); }
+  __device__ double* Sq(int k) const { return vecp(PV_FIXED + 3 * P.maxd + k); }
+  __device__ double* Sg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + P.nslots + k); }
+
+  // ------------------------------------------------------------- metric
+  __device__ double mi(int i) const { return METRIC == METRIC_UNIT ? 1.0 : minv[i]; }
+  __device__ double sharp(const double* x, int i) const { return METRIC == METRIC_UNIT ? x[i] : minv[i] * x[i]; }
+
+  __device__ double kinetic(const double* p) const {
+    double a = 0;
+    for (int i = lane; i < D; i += NUTS_WAVE) a += p[i] * sharp(p, i);
+    return 0.5 * wave_sum(a);
+  }
+
+  // ------------------------------------------------------------- draws
+  __device__ double uniform() {
+    const uint64_t k = s.uctr++;
+    if (B.inj_u) return k < (uint64_t)B.n_inj_u ? B.inj_u[(int64_t)c * B.n_inj_u + k] : 0.5;
+    return stream_uniform(P.seed, uid, STREAM_UNIFORM, k);
+  }
+  // diag_e_metric::sample_p / unit_e_metric::sample_p
+  __device__ void sample_p(double* p) {
+    if (B.inj_z) {
+      const double* z = B.inj_z + (int64_t)c * B.n_inj_z;
+      for (int i = lane; i < D; i += NUTS_WAVE) {
+        const uint64_t k = s.nctr + (uint64_t)i;
+        const double zi = k < (uint64_t)B.n_inj_z ? z[k] : 0.0;
+        p[i] = METRIC == METRIC_UNIT ? zi : zi / sqrt(minv[i]);
+      }
+      s.nctr += D;
+      return;
+    }
+    for (int i = lane; i < D; i += NUTS_WAVE) {
+      const double z = stream_normal(P.seed, uid, s.nctr + (uint64_t)(i >> 1), i & 1);
+      p[i] = METRIC == METRIC_UNIT ? z : z / sqrt(minv[i]);
+    }
+    s.nctr += (uint64_t)((D + 1) / 2);
+  }
+
+  // ------------------------------------------------------------- vectors
+  __device__ void copy(double* dst, const double* src) const {
+    for (int i = lane; i < D; i += NUTS_WAVE) dst[i] = src[i];
+  }
+  __device__ void swap(double* a, double* b) const {
+    for (int i = lane; i < D; i += NUTS_WAVE) { double t = a[i]; a[i] = b[i]; b[i] = t; }
+  }
+  // expl_leapfrog: begin_update_p(eps/2), update_q(eps)
+  __device__ void leap_begin(double e) const {
+    double* p = vecp(PV_P);
+    const double h = 0.5 * e;
+    for (int i = lane; i < D; i += NUTS_WAVE) {
+      const double pi = p[i] - h * g[i];
+      p[i] = pi;
+      q[i] = q[i] + e * (mi(i) * pi);
+    }
+  }
+  // end_update_p(eps/2)
+  __device__ void leap_end(double e) const {
+    double* p = vecp(PV_P);
+    const double h = 0.5 * e;
+    for (int i = lane; i < D; i += NUTS_WAVE) p[i] = p[i] - h * g[i];
+  }
+
+  // ------------------------------------------------------------- slots
+  __device__ int alloc_slot() {
+    const uint32_t m = s.free_mask;
+    const int k = __builtin_ctz(m);  // a free slot always exists (<= maxd+1 live)
+    s.free_mask = m & ~(1u << k);
+    return k;
+  }
+  __device__ void free_slot(int k) { if (k >= 0) s.free_mask |= (1u << k); }
+  __device__ int store_leaf_slot() {
+    const int k = alloc_slot();
+    copy(Sq(k), q);
+    copy(Sg(k), g);
+    s.slot_V[k] = s.Vz;
+    s.slot_H[k] = s.leaf_h;
+    return k;
+  }
+
+  // -------------------------------------------------------- control flow
+  __device__ bool request(int next) {
+    s.phase = next;
+    s.n_grad++;
+    return true;
+  }
+  __device__ void stop(int phase) {
+    s.phase = phase;
+    if (lane == 0) atomicAdd(B.n_stopped, 1);
+  }
+  __device__ void fail(int code) {
+    s.failure = code;
+    stop(PH_FAILED);
+  }
+
+  // services::util::initialize
+  __device__ bool init_try() {
+    const int t = s.init_tries;
+    for (int i = lane; i < D; i += NUTS_WAVE) {
+      double v;
+      if (B.user_init) v = B.user_init[B.off[c] + i];
+      else if (P.init_radius == 0) v = 0.0;
+      else v = P.init_radius * (2.0 * stream_uniform(P.seed, uid, STREAM_INIT, (uint64_t)t * (uint64_t)D + i) - 1.0);
+      q[i] = v;
+    }
+    return request(PH_INIT_CHECK);
+  }
+  __device__ bool init_check() {
+    const double V = B.V[c];
+    int ok = 1;
+    for (int i = lane; i < D; i += NUTS_WAVE) ok &= isfinite(g[i]) ? 1 : 0;
+    ok = wave_and(ok) && isfinite(V);
+    if (ok) {
+      const int k = s.sample_slot;
+      copy(Sq(k), q);
+      copy(Sg(k), g);
+      s.slot_V[k] = V;
+      s.Vz = V;
+      if (P.adapt) { s.ss_return = SS_RET_TRANS; s.phase = PH_SS_START; }
+      else s.phase = PH_TRANS_BEGIN;
+    } else {
+      s.init_tries++;
+      if (s.init_tries >= P.max_init_tries) fail(FAIL_INIT);
+      else s.phase = PH_INIT_TRY;
+    }
+    return false;
+  }
+
+  // base_hmc::init_stepsize
+  __device__ bool ss_probe_begin() {
+    const int k = s.sample_slot;
+    copy(q, Sq(k));
+    copy(g, Sg(k));
+    s.Vz = s.slot_V[k];
+    double* p = vecp(PV_P);
+    sample_p(p);
+    s.ss_H0 = kinetic(p) + s.Vz;
+    leap_begin(s.nom_eps);
+    return request(PH_SS_PROBE);
+  }
+  __device__ bool ss_start() {
+    const double e = s.nom_eps;
+    if (e == 0 || e > 1e7 || isnan(e)) { ss_finish(); return false; }
+    s.ss_first = 1;
+    return ss_probe_begin();
+  }
+  __device__ bool ss_probe() {
+    s.Vz = B.V[c];
+    leap_end(s.nom_eps);
+    double h = kinetic(vecp(PV_P)) + s.Vz;
+    if (isnan(h)) h = __builtin_inf();
+    const double dH = s.ss_H0 - h;
+    const double l08 = log(0.8);
+    if (s.ss_first) {
+      s.ss_dir = dH > l08 ? 1 : -1;
+      s.ss_first = 0;
+      return ss_probe_begin();
+    }
+    if ((s.ss_dir == 1) && !(dH > l08)) { ss_finish(); return false; }
+    if ((s.ss_dir == -1) && !(dH < l08)) { ss_finish(); return false; }
+    s.nom_eps = s.ss_dir == 1 ? s.nom_eps * 2 : s.nom_eps * 0.5;
+    if (s.nom_eps > 1e7) { fail(FAIL_STEPSIZE_IMPROPER); return false; }
+    if (s.nom_eps == 0) { fail(FAIL_STEPSIZE_ZERO); return false; }
+    return ss_probe_begin();
+  }
+  __device__ void ss_finish() {
+    if (s.ss_return == SS_RET_WINDOW) {
+      s.sa_mu = log(10 * s.nom_eps);
+      s.sa_counter = 0; s.sa_sbar = 0; s.sa_xbar = 0;
+      iteration_next();
+    } else {
+      s.phase = PH_TRANS_BEGIN;
+    }
+  }
+
+  // base_nuts::transition, set-up part
+  __device__ void trans_begin() {
+    if (s.iter >= P.iter_target) { stop(PH_PAUSED); return; }
+    s.eps = s.nom_eps;
+    if (P.jitter != 0) s.eps *= 1.0 + P.jitter * (2.0 * uniform() - 1.0);
+    const int k = s.sample_slot;
+    // every slot but the current draw's is free at the start of a transition
+    // (a transition that ended inside a subtree leaves its checkpoints behind)
+    const uint32_t all = P.nslots >= 32 ? 0xFFFFFFFFu : ((1u << P.nslots) - 1u);
+    s.free_mask = all & ~(1u << k);
+    double* p = vecp(PV_P);
+    copy(q, Sq(k));
+    copy(g, Sg(k));
+    s.Vz = s.slot_V[k];
+    sample_p(p);
+    double* oq = vecp(PV_OQ); double* op = vecp(PV_OP); double* og = vecp(PV_OG);
+    double* pfb = vecp(PV_PFB); double* pbf = vecp(PV_PBF); double* rho = vecp(PV_RHO);
+    for (int i = lane; i < D; i += NUTS_WAVE) {
+      const double pi = p[i];
+      oq[i] = q[i]; op[i] = pi; og[i] = g[i];
+      pfb[i] = pi; pbf[i] = pi; rho[i] = pi;
+    }
+    s.Vo = s.Vz;
+    s.z_is_fwd = 1;
+    s.log_w = 0;
+    s.H0 = kinetic(p) + s.Vz;
+    s.slot_H[k] = s.H0;
+    s.n_leapfrog = 0;
+    s.sum_metro = 0;
+    s.depth = 0;
+    s.divergent = 0;
+    s.phase = PH_DOUBLING_BEGIN;
+  }
+
+  __device__ bool doubling_begin() {
+    const int fwd = uniform() > 0.5 ? 1 : 0;
+    if (fwd != s.z_is_fwd) {
+      swap(q, vecp(PV_OQ));
+      swap(vecp(PV_P), vecp(PV_OP));
+      swap(g, vecp(PV_OG));
+      const double t = s.Vz; s.Vz = s.Vo; s.Vo = t;
+      s.z_is_fwd = fwd;
+    }
+    // p_bck_fwd = p_fwd_bck (forward) / p_fwd_bck = p_bck_fwd (backward)
+    if (fwd) copy(vecp(PV_PBF), vecp(PV_PFB));
+    else copy(vecp(PV_PFB), vecp(PV_PBF));
+    s.dir = fwd ? 1 : -1;
+    s.leaf_idx = 0;
+    leap_begin(s.dir * s.eps);
+    return request(PH_LEAF);
+  }
+
+  // build_tree base case plus the merges that close at this leaf
+  __device__ bool leaf() {
+    const double e = s.dir * s.eps;
+    s.Vz = B.V[c];
+    leap_end(e);
+    double* p = vecp(PV_P);
+    s.n_leapfrog++;
+    double h = kinetic(p) + s.Vz;
+    if (isnan(h)) h = __builtin_inf();
+    if ((h - s.H0) > P.max_deltaH) s.divergent = 1;
+    const double lw = s.H0 - h;
+    if (s.H0 - h > 0) s.sum_metro += 1;
+    else s.sum_metro += exp(s.H0 - h);
+    s.leaf_h = h;
+    if (s.leaf_idx == 0) copy(s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF), p);
+    if (s.divergent) { trans_end(); return false; }
+
+    double* srho = vecp(PV_SRHO);
+    copy(srho, p);
+    s.S_logw = lw;
+    s.s_prop = PROP_LEAF;
+    int k = 0;
+    while ((s.leaf_idx >> k) & 1) {
+      const double lsw_sub = log_sum_exp2(s.L_logw[k], s.S_logw);
+      bool take;
+      if (s.S_logw > lsw_sub) take = true;
+      else take = uniform() < exp(s.S_logw - lsw_sub);
+      if (take) free_slot(s.L_prop[k]);
+      else { free_slot(s.s_prop); s.s_prop = s.L_prop[k]; }
+      const double* lb = Lbeg(k);
+      const double* le = Lend(k);
+      const double* lr = Lrho(k);
+      const double* sb = k == 0 ? p : Lbeg(k - 1);
+      double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+      for (int i = lane; i < D; i += NUTS_WAVE) {
+        const double rs = lr[i] + srho[i];
+        const double r2 = lr[i] + sb[i];
+        const double r3 = srho[i] + le[i];
+        const double slb = sharp(lb, i), sp = sharp(p, i);
+        a1 += slb * rs; b1 += sp * rs;
+        a2 += slb * r2; b2 += sharp(sb, i) * r2;
+        a3 += sharp(le, i) * r3; b3 += sp * r3;
+        srho[i] = rs;
+      }
+      a1 = wave_sum(a1); b1 = wave_sum(b1); a2 = wave_sum(a2);
+      b2 = wave_sum(b2); a3 = wave_sum(a3); b3 = wave_sum(b3);
+      const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
+      s.S_logw = lsw_sub;
+      if (!ok) { free_slot(s.s_prop); trans_end(); return false; }
+      ++k;
+    }
+    if (s.leaf_idx == (1 << s.depth) - 1) { doubling_end(); return false; }
+    // the closed subtree becomes the open left sibling at level k
+    copy(Lbeg(k), k == 0 ? p : Lbeg(k - 1));
+    copy(Lend(k), p);
+    copy(Lrho(k), srho);
+    s.L_logw[k] = s.S_logw;
+    s.L_prop[k] = s.s_prop == PROP_LEAF ? store_leaf_slot() : s.s_prop;
+    s.leaf_idx++;
+    leap_begin(e);
+    return request(PH_LEAF);
+  }
+
+  // top-level merge of a complete, valid subtree (biased progressive sampling)
+  __device__ void doubling_end() {
+    s.depth++;
+    bool take;
+    if (s.S_logw > s.log_w) take = true;
+    else take = uniform() < exp(s.S_logw - s.log_w);
+    if (take) {
+      const int ns = s.s_prop == PROP_LEAF ? store_leaf_slot() : s.s_prop;
+      free_slot(s.sample_slot);
+      s.sample_slot = ns;
+    } else {
+      free_slot(s.s_prop);
+    }
+    s.log_w = log_sum_exp2(s.log_w, s.S_logw);
+    const int fwd = s.z_is_fwd;
+    const double* p = vecp(PV_P);
+    const double* op = vecp(PV_OP);
+    const double* pfb = vecp(PV_PFB);
+    const double* pbf = vecp(PV_PBF);
+    const double* srho = vecp(PV_SRHO);
+    double* rho = vecp(PV_RHO);
+    const double* bb = fwd ? op : p;   // p_bck_bck
+    const double* ff = fwd ? p : op;   // p_fwd_fwd
+    double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+    for (int i = lane; i < D; i += NUTS_WAVE) {
+      const double old = rho[i], nw = srho[i];
+      const double rn = fwd ? old + nw : nw + old;  // rho_bck + rho_fwd
+      const double rb = fwd ? old : nw, rf = fwd ? nw : old;
+      const double r2 = rb + pfb[i];
+      const double r3 = rf + pbf[i];
+      const double sbb = sharp(bb, i), sff = sharp(ff, i);
+      a1 += sbb * rn; b1 += sff * rn;
+      a2 += sbb * r2; b2 += sharp(pfb, i) * r2;
+      a3 += sharp(pbf, i) * r3; b3 += sff * r3;
+      rho[i] = rn;
+    }
+    a1 = wave_sum(a1); b1 = wave_sum(b1); a2 = wave_sum(a2);
+    b2 = wave_sum(b2); a3 = wave_sum(a3); b3 = wave_sum(b3);
+    const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
+    if (!ok || s.depth >= P.max_depth) trans_end();
+    else s.phase = PH_DOUBLING_BEGIN;
+  }
+
+  // window logic of stan::mcmc::windowed_adaptation (unsigned arithmetic)
+  __device__ bool in_window() const {
+    return (s.w_counter >= P.w_init_buffer) && (s.w_counter < P.w_num_warmup - P.w_term_buffer) &&
+           (s.w_counter != P.w_num_warmup);
+  }
+  __device__ bool end_window() const { return (s.w_counter == s.w_next) && (s.w_counter != P.w_num_warmup); }
+  __device__ void next_window() {
+    if (s.w_next == P.w_num_warmup - P.w_term_buffer - 1) return;
+    s.w_size *= 2;
+    s.w_next = s.w_counter + s.w_size;
+    if (s.w_next != P.w_num_warmup - P.w_term_buffer - 1) {
+      const uint32_t boundary = s.w_next + 2 * s.w_size;
+      if (boundary >= P.w_num_warmup - P.w_term_buffer) s.w_next = P.w_num_warmup - P.w_term_buffer - 1;
+    }
+  }
+  // var_adaptation::learn_variance; returns true at a window end
+  __device__ bool learn_variance(const double* qs) {
+    double* wm = vecp(PV_WM);
+    double* wm2 = vecp(PV_WM2);
+    if (in_window()) {
+      s.welford_n++;
+      const double n = (double)s.welford_n;
+      for (int i = lane; i < D; i += NUTS_WAVE) {
+        const double delta = qs[i] - wm[i];
+        const double m = wm[i] + delta / n;
+        wm[i] = m;
+        wm2[i] = wm2[i] + (qs[i] - m) * delta;
+      }
+    }
+    if (end_window()) {
+      next_window();
+      const double n = (double)s.welford_n;
+      int ok = 1;
+      for (int i = lane; i < D; i += NUTS_WAVE) {
+        const double var = s.welford_n > 1 ? wm2[i] / (n - 1.0) : minv[i];
+        const double v = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0));
+        minv[i] = v;
+        ok &= isfinite(v) ? 1 : 0;
+        wm[i] = 0; wm2[i] = 0;
+      }
+      s.welford_n = 0;
+      s.w_counter++;
+      if (!wave_and(ok)) { fail(FAIL_METRIC_OVERFLOW); return false; }
+      return true;
+    }
+    s.w_counter++;
+    return false;
+  }
+
+  // end of base_nuts::transition + adapt_diag_e_nuts::transition + writer
+  __device__ void trans_end() {
+    const double accept = s.sum_metro / (double)s.n_leapfrog;
+    const int k = s.sample_slot;
+    s.n_leapfrog_total += s.n_leapfrog;
+    const bool warm = s.iter < P.num_warmup;
+    const bool save = warm ? (P.save_warmup && (s.iter % P.thin) == 0) : (((s.iter - P.num_warmup) % P.thin) == 0);
+    if (save) {
+      double* row = B.draws + B.draw_off[c] + (int64_t)s.draw_idx * (7 + D);
+      if (lane == 0) {
+        row[0] = -s.slot_V[k];
+        row[1] = accept;
+        row[2] = s.eps;
+        row[3] = s.depth;
+        row[4] = s.n_leapfrog;
+        row[5] = s.divergent;
+        row[6] = s.slot_H[k];
+      }
+      copy(row + 7, Sq(k));
+      s.draw_idx++;
+    }
+    if (s.adapt_flag) {
+      // stepsize_adaptation::learn_stepsize
+      s.sa_counter += 1;
+      const double a = accept > 1 ? 1 : accept;
+      const double eta = 1.0 / (s.sa_counter + P.t0);
+      s.sa_sbar = (1.0 - eta) * s.sa_sbar + eta * (P.delta - a);
+      const double x = s.sa_mu - s.sa_sbar * sqrt(s.sa_counter) / P.gamma;
+      const double x_eta = pow(s.sa_counter, -P.kappa);
+      s.sa_xbar = (1.0 - x_eta) * s.sa_xbar + x_eta * x;
+      s.nom_eps = exp(x);
+      if (METRIC != METRIC_UNIT && P.metric_adapt) {
+        const bool update = learn_variance(Sq(k));
+        if (s.phase == PH_FAILED) return;
+        if (update) { s.ss_return = SS_RET_WINDOW; s.phase = PH_SS_START; return; }
+      }
+    }
+    iteration_next();
+  }
+
+  __device__ void iteration_next() {
+    s.iter++;
+    if (s.iter == P.num_warmup && s.adapt_flag) {
+      s.adapt_flag = 0;
+      s.nom_eps = exp(s.sa_xbar);  // stepsize_adaptation::complete_adaptation
+    }
+    if (s.iter >= P.num_warmup + P.num_samples) stop(PH_DONE);
+    else s.phase = PH_TRANS_BEGIN;
+  }
+
+  // One step of the machine; true when V, G at Q are needed next.
+  __device__ bool dispatch() {
+    switch (s.phase) {
+      case PH_INIT_TRY: return init_try();
+      case PH_INIT_CHECK: return init_check();
+      case PH_SS_START: return ss_start();
+      case PH_SS_PROBE: return ss_probe();
+      case PH_TRANS_BEGIN: trans_begin(); return false;
+      case PH_DOUBLING_BEGIN: return doubling_begin();
+      case PH_LEAF: return leaf();
+      default: return false;
+    }
+  }
+
+  __device__ void run() {
+    int budget = GradFn::fused ? P.fused_budget : 1;
+    for (;;) {
+      if (s.phase >= PH_PAUSED) return;
+      const bool req = dispatch();
+      if (!req) continue;
+      if (!GradFn::fused) { if (lane == 0) B.need[c] = 1; return; }
+      const double V = grad(c, q, g, D, lane);
+      if (lane == 0) B.V[c] = V;
+      if (--budget == 0) return;
+    }
+  }
+};
+
+// Stage ChainState in LDS, run, write back.  One wave per chain; 4 waves per block.
+template <int METRIC, class GradFn>
+__device__ __forceinline__ void nuts_advance_body(const NutsParams& P, const NutsBuffers& B, const GradFn& grad) {
+  __shared__ ChainState sst[4];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + w;
+  if (c >= B.C) return;
+  ChainState& s = sst[w];
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(B.st + c);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&s);
+    for (int i = lane; i < (int)(sizeof(ChainState) / 4); i += 64) dst[i] = src[i];
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (s.phase < PH_PAUSED) {
+    NutsWave<METRIC, GradFn> wv(P, B, s, grad, c, lane);
+    wv.run();
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  {
+    uint32_t* dst = reinterpret_cast<uint32_t*>(B.st + c);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&s);
+    for (int i = lane; i < (int)(sizeof(ChainState) / 4); i += 64) dst[i] = src[i];
+  }
+}
+
+}  // namespace nutsgpu

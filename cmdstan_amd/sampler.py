@@ -1,0 +1,211 @@
+"""Python mirror of the sampling service interface.
+
+``Model(family, data)`` plays the role of CmdStan's compiled model plus its
+``data file=`` context (src/cmdstan/command.hpp:170-171); ``Model.sample``
+takes the arguments of ``sample ... algorithm=hmc engine=nuts`` with CmdStan's
+names and defaults (src/cmdstan/arguments/arg_sample.hpp, arg_adapt.hpp,
+arg_hmc.hpp, arg_nuts.hpp, arg_metric.hpp) and calls ``nuts_sample`` of the C
+ABI; ``Model.log_prob_grad`` is the ``log_prob`` method
+(src/cmdstan/command_helper.hpp:569-600).
+"""
+import ctypes as C
+import json
+
+import numpy as np
+
+from ._lib import DP, I64P, last_error, lib, nuts_config, nuts_output
+
+SAMPLER_COLUMNS = ["lp__", "accept_stat__", "stepsize__", "treedepth__", "n_leapfrog__", "divergent__", "energy__"]
+METRICS = {"unit_e": 0, "diag_e": 1, "dense_e": 2}
+
+
+class SamplerError(RuntimeError):
+    pass
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(DP)
+
+
+class Data:
+    """A var_context: named arrays (from a dict or CmdStan JSON text)."""
+
+    def __init__(self, values=None, json_text=None):
+        L = lib()
+        self.h = L.nuts_data_create()
+        if json_text is not None:
+            if L.nuts_data_parse_json(self.h, json_text.encode()) != 0:
+                raise SamplerError(last_error())
+        for k, v in (values or {}).items():
+            a = np.asarray(v)
+            is_int = np.issubdtype(a.dtype, np.integer) or a.dtype == bool
+            a = _f64(a)
+            dims = (C.c_int64 * max(1, a.ndim))(*a.shape)
+            if L.nuts_data_add(self.h, k.encode(), a.ndim, dims, _dp(a), int(is_int)) != 0:
+                raise SamplerError(last_error())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().nuts_data_destroy(self.h)
+
+
+class Fit:
+    """Draws and adaptation results of one sampling run."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def chain_draws(self, k):
+        return self.draws[k]
+
+    def param(self, name, dataset=0):
+        cols = self.column_names[dataset]
+        j = cols.index(name)
+        ks = [k for k in range(len(self.draws)) if self.dataset_of_chain[k] == dataset]
+        return np.stack([self.draws[k][:, j] for k in ks], axis=1)  # draws x chains
+
+
+class Model:
+    def __init__(self, family, data):
+        datasets = data if isinstance(data, (list, tuple)) else [data]
+        self._data = [d if isinstance(d, Data) else (Data(json_text=d) if isinstance(d, str) else Data(d))
+                      for d in datasets]
+        arr = (C.c_void_p * len(self._data))(*[d.h for d in self._data])
+        self.h = lib().nuts_model_create(family.encode(), arr, len(self._data))
+        if not self.h:
+            raise SamplerError(last_error())
+        self.family = family
+        self.num_datasets = len(self._data)
+        self.dims, self.num_outputs, self.param_names, self.unconstrained_names = [], [], [], []
+        buf = C.create_string_buffer(256)
+        for m in range(self.num_datasets):
+            d, o = C.c_int(), C.c_int()
+            lib().nuts_model_dims(self.h, m, C.byref(d), C.byref(o))
+            self.dims.append(d.value)
+            self.num_outputs.append(o.value)
+            names = []
+            for i in range(o.value):
+                lib().nuts_model_param_name(self.h, m, i, buf, 256)
+                names.append(buf.value.decode())
+            self.param_names.append(names)
+            un = []
+            for i in range(d.value):
+                lib().nuts_model_unconstrained_name(self.h, m, i, buf, 256)
+                un.append(buf.value.decode())
+            self.unconstrained_names.append(un)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().nuts_model_destroy(self.h)
+
+    @property
+    def dim(self):
+        return self.dims[0]
+
+    def log_prob_grad(self, q, jacobian=True, dataset=0):
+        q = _f64(np.atleast_2d(q))
+        n, D = q.shape
+        if D != self.dims[dataset]:
+            raise ValueError("dimension mismatch")
+        lp = np.zeros(n)
+        g = np.zeros((n, D))
+        if lib().nuts_log_prob_grad(self.h, dataset, _dp(q), n, int(jacobian), _dp(lp), _dp(g)) != 0:
+            raise SamplerError(last_error())
+        return lp, g
+
+    def sample(self, num_chains=1, num_samples=1000, num_warmup=1000, save_warmup=False, thin=1,
+               adapt_engaged=True, delta=0.8, gamma=0.05, kappa=0.75, t0=10.0, init_buffer=75, term_buffer=50,
+               window=25, max_depth=10, metric="diag_e", stepsize=1.0, stepsize_jitter=0.0, id=1, seed=0,
+               init=2.0, inv_metric=None, fused=-1):
+        cfg = nuts_config()
+        lib().nuts_config_default(C.byref(cfg))
+        cfg.num_samples, cfg.num_warmup, cfg.save_warmup, cfg.thin = num_samples, num_warmup, int(save_warmup), thin
+        cfg.adapt_engaged, cfg.delta, cfg.gamma, cfg.kappa, cfg.t0 = int(adapt_engaged), delta, gamma, kappa, t0
+        cfg.init_buffer, cfg.term_buffer, cfg.window = init_buffer, term_buffer, window
+        cfg.max_depth, cfg.metric = max_depth, METRICS[metric] if isinstance(metric, str) else metric
+        cfg.stepsize, cfg.stepsize_jitter = stepsize, stepsize_jitter
+        cfg.num_chains, cfg.id, cfg.seed, cfg.fused = num_chains, id, seed, fused
+        init_arr = None
+        if isinstance(init, (int, float)):
+            cfg.init_radius = float(init)
+        else:
+            init_arr = _f64(np.broadcast_to(np.asarray(init, dtype=np.float64),
+                                            (num_chains * self.num_datasets, self.dims[0])))
+        im = None
+        C_tot = num_chains * self.num_datasets
+        if inv_metric is not None:
+            im = _f64(np.broadcast_to(np.asarray(inv_metric, dtype=np.float64).ravel(),
+                                      (C_tot, np.asarray(inv_metric).size)))
+        if lib().nuts_config_validate(C.byref(cfg)) != 0:
+            raise SamplerError(last_error())
+        rows = lib().nuts_output_rows(C.byref(cfg))
+        ds_of_chain = [m for m in range(self.num_datasets) for _ in range(num_chains)]
+        cols = [7 + self.num_outputs[m] for m in ds_of_chain]
+        total = sum(rows * c for c in cols)
+        draws = np.zeros(total)
+        step = np.zeros(C_tot)
+        msz = sum(self.dims[m] for m in ds_of_chain)
+        imet = np.zeros(msz)
+        nlf = np.zeros(C_tot, dtype=np.int64)
+        nge = np.zeros(C_tot, dtype=np.int64)
+        status = np.zeros(C_tot, dtype=np.int32)
+        out = nuts_output()
+        out.draws, out.stepsize, out.inv_metric = _dp(draws), _dp(step), _dp(imet)
+        out.n_leapfrog = nlf.ctypes.data_as(I64P)
+        out.grad_evals = nge.ctypes.data_as(I64P)
+        out.status = status.ctypes.data_as(C.POINTER(C.c_int))
+        rc = lib().nuts_sample(self.h, C.byref(cfg), _dp(init_arr), _dp(im), C.byref(out))
+        err = last_error() if rc != 0 else ""
+        per_chain, o, mo, metrics = [], 0, 0, []
+        for k in range(C_tot):
+            n = rows * cols[k]
+            per_chain.append(draws[o:o + n].reshape(rows, cols[k]))
+            o += n
+            D = self.dims[ds_of_chain[k]]
+            metrics.append(imet[mo:mo + D])
+            mo += D
+        fit = Fit(draws=per_chain, stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
+                  status=status, warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds,
+                  leaf_steps=out.leaf_steps, grad_seconds_per_launch=out.grad_seconds_per_launch,
+                  grad_launches_timed=out.grad_launches_timed, column_names=[SAMPLER_COLUMNS + p for p in self.param_names],
+                  dataset_of_chain=ds_of_chain, config=cfg, error=err)
+        if rc != 0:
+            raise SamplerError(err)
+        return fit
+
+    def transition_replay(self, q0, uniforms, normals, stepsize, metric="diag_e", inv_metric=None, jitter=0.0,
+                          max_depth=10, dataset=0):
+        q0 = _f64(np.atleast_2d(q0))
+        nch, D = q0.shape
+        u = _f64(np.atleast_2d(uniforms))
+        z = _f64(np.atleast_2d(normals))
+        if u.shape[0] != nch or z.shape[0] != nch:
+            raise ValueError("one row of draws per chain")
+        m = METRICS[metric] if isinstance(metric, str) else metric
+        im = _f64(np.broadcast_to(np.ones(D) if inv_metric is None else np.asarray(inv_metric, dtype=np.float64),
+                                  (nch, D)))
+        out_q = np.zeros((nch, D))
+        info = np.zeros((nch, 9))
+        rc = lib().nuts_transition_replay(self.h, dataset, nch, m, stepsize, jitter, max_depth, _dp(im), _dp(q0),
+                                          _dp(u), u.shape[1], _dp(z), z.shape[1], _dp(out_q), _dp(info))
+        if rc != 0:
+            raise SamplerError(last_error())
+        keys = ["depth", "n_leapfrog", "divergent", "accept_stat", "energy", "lp", "uniforms_used", "normals_used",
+                "stepsize"]
+        return out_q, [dict(zip(keys, r.tolist())) for r in info]
+
+
+def read_json_file(path):
+    with open(path) as f:
+        return f.read()
+
+
+def data_from_json(path):
+    return Data(json_text=read_json_file(path))
+
+
+__all__ = ["Model", "Data", "Fit", "SamplerError", "SAMPLER_COLUMNS", "json", "data_from_json"]

@@ -1,0 +1,164 @@
+/* cmdstan_amd — batched NUTS on AMD Instinct MI355X (gfx950) behind a C ABI.
+ *
+ * The entry points replace, for `method=sample algorithm=hmc engine=nuts`, the
+ * calls CmdStan's command layer makes into the (absent) stan services library:
+ *
+ *   nuts_data_*          stan::io::var_context built from `data file=`
+ *                        (src/cmdstan/command_helper.hpp:116-133 get_var_context)
+ *   nuts_model_create    new_model(var_context&, seed, ostream*)
+ *                        (src/cmdstan/command.hpp:71-72, 170-171)
+ *   nuts_sample          stan::services::sample::hmc_nuts_{unit,diag,dense}_e[_adapt]
+ *                        (src/cmdstan/command.hpp:603-697; diag_e + adapt at :681-688)
+ *   nuts_log_prob_grad   services_log_prob_grad -> stan::model::log_prob_grad<true, jacobian>
+ *                        (src/cmdstan/command_helper.hpp:569-600)
+ *   nuts_last_error      the message CmdStan prints before returning
+ *                        return_codes::NOT_OK (src/cmdstan/main.cpp:4-15)
+ *
+ * Conventions: plain pointers to caller-owned host memory; device memory is
+ * owned by the library.  Every function returns NUTS_OK (0) or NUTS_NOT_OK (1)
+ * and leaves a message for nuts_last_error().  One handle per host thread.
+ */
+#ifndef CMDSTAN_AMD_H
+#define CMDSTAN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUTS_OK 0
+#define NUTS_NOT_OK 1
+
+#define NUTS_METRIC_UNIT_E 0
+#define NUTS_METRIC_DIAG_E 1
+#define NUTS_METRIC_DENSE_E 2
+
+typedef struct nuts_data nuts_data;
+typedef struct nuts_model nuts_model;
+
+/* ---------------------------------------------------------------- data */
+/* A named-array context: stan::io::var_context (command_helper.hpp:116-133). */
+nuts_data* nuts_data_create(void);
+void nuts_data_destroy(nuts_data* d);
+/* Adds a real (is_int=0) or integer (is_int=1) array, row-major, ndim 0 = scalar. */
+int nuts_data_add(nuts_data* d, const char* name, int ndim, const int64_t* dims, const double* values, int is_int);
+/* Parses CmdStan JSON data text (json_data_handler semantics: numbers, nested
+ * arrays, "inf"/"-inf"/"NaN" strings). */
+int nuts_data_parse_json(nuts_data* d, const char* json_text);
+int nuts_data_get(const nuts_data* d, const char* name, int* ndim, int64_t* dims /* >= 8 */, const double** values);
+
+/* --------------------------------------------------------------- model */
+/* Creates a model of a registered family for one or more datasets (a batch of
+ * models of uneven size sampled in one launch).  Data are copied to HBM here.
+ * Families: bernoulli, logistic_regression, eight_schools_cp, hier_normal_nc,
+ * hier_logistic_cp, gauss. */
+nuts_model* nuts_model_create(const char* family, const nuts_data* const* datasets, int num_datasets);
+void nuts_model_destroy(nuts_model* m);
+int nuts_model_num_datasets(const nuts_model* m);
+/* Unconstrained dimension and number of constrained outputs of a dataset. */
+int nuts_model_dims(const nuts_model* m, int dataset, int* num_unconstrained, int* num_outputs);
+/* Name of output column i (constrained parameters in Stan CSV order, e.g. "theta.1"). */
+int nuts_model_param_name(const nuts_model* m, int dataset, int i, char* buf, int buflen);
+/* Name of unconstrained coordinate i (log_prob output "g_<name>"). */
+int nuts_model_unconstrained_name(const nuts_model* m, int dataset, int i, char* buf, int buflen);
+
+/* ------------------------------------------------------------ log_prob */
+/* lp and its gradient at n_points unconstrained points of one dataset, computed
+ * on the GPU in FP64 (propto=true; jacobian=0/1).  q, grad: [n_points][D]. */
+int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points, int jacobian, double* lp,
+                       double* grad);
+
+/* -------------------------------------------------------------- sample */
+/* Argument tree of `sample ... algorithm=hmc engine=nuts` with CmdStan defaults
+ * (arg_sample.hpp:18-29, arg_adapt.hpp:19-37, arg_hmc.hpp:19-27,
+ * arg_nuts.hpp:15-16, arg_metric.hpp:17-22, arg_id.hpp, arg_seed.hpp, arg_init.hpp). */
+typedef struct nuts_config {
+  int num_samples;        /* 1000 */
+  int num_warmup;         /* 1000 */
+  int save_warmup;        /* 0 */
+  int thin;               /* 1 */
+  int adapt_engaged;      /* 1 */
+  double gamma;           /* 0.05 */
+  double delta;           /* 0.8 */
+  double kappa;           /* 0.75 */
+  double t0;              /* 10 */
+  int init_buffer;        /* 75 */
+  int term_buffer;        /* 50 */
+  int window;             /* 25 */
+  int max_depth;          /* 10 */
+  int metric;             /* NUTS_METRIC_DIAG_E */
+  double stepsize;        /* 1 */
+  double stepsize_jitter; /* 0 */
+  int num_chains;         /* chains per dataset, 1 */
+  int id;                 /* 1: chain k uses RNG stream id + k */
+  uint64_t seed;          /* random seed */
+  double init_radius;     /* 2 */
+  int fused;              /* -1 auto, 0 lockstep (batched gradient kernels), 1 fused per-chain */
+} nuts_config;
+
+void nuts_config_default(nuts_config* cfg);
+/* Validates like CmdStan's command layer: e.g. adapt engaged with num_warmup=0
+ * is an error (command.hpp:567-574), delta in [0.1, 1), jitter in [0, 1]. */
+int nuts_config_validate(const nuts_config* cfg);
+/* Rows of each chain's draws block: saved warmup + sampling iterations. */
+int nuts_output_rows(const nuts_config* cfg);
+
+/* Caller-allocated outputs (any pointer may be NULL).  Chains are ordered by
+ * dataset, num_chains per dataset; chain k of dataset m is chain m*num_chains+k.
+ *   draws:     chains back to back, each rows x (7 + num_outputs of its dataset):
+ *              lp__, accept_stat__, stepsize__, treedepth__, n_leapfrog__,
+ *              divergent__, energy__, then the constrained parameters
+ *   stepsize:  [chains] adapted (or given) step size
+ *   inv_metric:[chains] blocks of D (diag_e/unit_e) or D*D (dense_e)
+ *   n_leapfrog:[chains] total leapfrog steps; grad_evals: [chains] all gradient
+ *              evaluations (initialisation and step-size search included)
+ *   status:    [chains] 0 ok, else a failure code (1 init, 2 improper posterior,
+ *              3 zero step size, 4 metric overflow) */
+typedef struct nuts_output {
+  double* draws;
+  double* stepsize;
+  double* inv_metric;
+  int64_t* n_leapfrog;
+  int64_t* grad_evals;
+  int* status;
+  double warmup_seconds;   /* filled by nuts_sample: GPU wall time of the warmup phase */
+  double sampling_seconds; /* ... and of the sampling phase */
+  int64_t leaf_steps;      /* lockstep gradient launches (0 in fused mode) */
+  double grad_seconds_per_launch; /* lockstep: mean duration of one batched gradient
+                                     launch in the sampling phase, from HIP events on
+                                     the sampler's stream (0 in fused mode) */
+  int64_t grad_launches_timed;
+} nuts_output;
+
+/* Runs warmup then sampling for every chain of every dataset in one batch.
+ * init: unconstrained inits [chains][D] or NULL (uniform(-R, R), R = init_radius);
+ * inv_metric: per-chain inverse metric [chains][D or D*D] or NULL (identity),
+ * the `metric_file=` input. */
+int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const double* inv_metric,
+                nuts_output* out);
+
+/* One NUTS transition per chain from q0 with injected uniforms and normals
+ * (consumed in the order of the upstream recursion).  No counterpart in the
+ * reference: this is the parity surface for the sampler arithmetic.
+ * q0: [chains][D]; u: [chains][n_u]; z: [chains][n_z]; inv_metric [chains][D];
+ * out_q: [chains][D]; info: [chains][9] = depth, n_leapfrog, divergent,
+ * accept_stat, energy, lp, uniforms used, normals used, step size. */
+int nuts_transition_replay(nuts_model* m, int dataset, int num_chains, int metric, double stepsize, double jitter,
+                           int max_depth, const double* inv_metric, const double* q0, const double* u, int64_t n_u,
+                           const double* z, int64_t n_z, double* out_q, double* info);
+
+/* ------------------------------------------------------------ summary */
+/* stansummary statistics (src/cmdstan/stansummary_helper.hpp:397-428 get_stats)
+ * for n_params parameters; draws[p][chain][draw].  out[p][10] = Mean, MCSE,
+ * StdDev, MAD, 5%, 50%, 95%, ESS_bulk, ESS_tail, R_hat.  Host code. */
+int nuts_summary(const double* draws, int n_draws, int n_chains, int n_params, int n_threads, double* out);
+
+const char* nuts_last_error(void);
+const char* nuts_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMDSTAN_AMD_H */

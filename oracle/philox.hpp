@@ -46,10 +46,11 @@ inline P4 philox4x32_10(P4 ctr, uint32_t k0, uint32_t k1) {
 // Purposes (4th counter word) keep the three streams of a chain independent.
 enum : uint32_t { PURPOSE_UNIFORM = 0u, PURPOSE_NORMAL = 1u, PURPOSE_INIT = 2u };
 
-// 53-bit uniform strictly inside (0,1): (k + 0.5) * 2^-53.
+// Uniform strictly inside (0,1) from 52 random bits: (k + 0.5) * 2^-52, exact
+// in binary64 (k + 0.5 needs 53 significant bits), so every platform agrees.
 inline double u53(uint32_t a, uint32_t b) {
-  uint64_t k = ((uint64_t)a << 21) | (uint64_t)(b >> 11);
-  return ((double)k + 0.5) * 1.1102230246251565404e-16;
+  uint64_t k = ((uint64_t)a << 20) | (uint64_t)(b >> 12);
+  return ((double)k + 0.5) * 2.220446049250313080847e-16;
 }
 
 inline P4 stream_block(uint64_t seed, uint32_t chain, uint32_t purpose, uint64_t idx) {

@@ -1,0 +1,105 @@
+"""Host-side checks of the C ABI (no GPU needed): the library loads, exports
+every symbol the header declares, parses CmdStan JSON data, validates the
+argument tree like CmdStan's command layer and reproduces stansummary's golden
+statistics."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cmdstan_amd as ca
+from cmdstan_amd._lib import SIGNATURES, nuts_config
+from cmdstan_amd.summary import summarize
+from tests.conftest import ROOT, load_stan_csv
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "cmdstan_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nuts_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ca.lib()
+    names = _declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in SIGNATURES, n
+
+
+def test_json_data(golden):
+    d = ca.Data(json_text=open(f"{golden}/bernoulli.data.json").read())
+    L = ca.lib()
+    nd = C.c_int()
+    dims = (C.c_int64 * 8)()
+    vals = C.POINTER(C.c_double)()
+    assert L.nuts_data_get(d.h, b"y", C.byref(nd), dims, C.byref(vals)) == 0
+    assert nd.value == 1 and dims[0] == 10
+    assert [vals[i] for i in range(10)] == [0, 1, 0, 0, 0, 0, 0, 0, 0, 1]
+    assert L.nuts_data_get(d.h, b"N", C.byref(nd), dims, C.byref(vals)) == 0
+    assert nd.value == 0 and vals[0] == 10
+    d2 = ca.Data(json_text='{"a": [[1.5, 2], [3, "inf"]], "b": []}')
+    assert L.nuts_data_get(d2.h, b"a", C.byref(nd), dims, C.byref(vals)) == 0
+    assert nd.value == 2 and dims[0] == 2 and dims[1] == 2 and vals[3] == np.inf
+    with pytest.raises(ca.SamplerError, match="JSON"):
+        ca.Data(json_text='{"a": [1, 2')
+    with pytest.raises(ca.SamplerError, match="ragged"):
+        ca.Data(json_text='{"a": [[1, 2], [3]]}')
+
+
+def _cfg(**kw):
+    c = nuts_config()
+    ca.lib().nuts_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def test_config_defaults_match_cmdstan():
+    c = _cfg()
+    assert (c.num_samples, c.num_warmup, c.save_warmup, c.thin) == (1000, 1000, 0, 1)
+    assert (c.adapt_engaged, c.gamma, c.delta, c.kappa, c.t0) == (1, 0.05, 0.8, 0.75, 10)
+    assert (c.init_buffer, c.term_buffer, c.window, c.max_depth, c.metric) == (75, 50, 25, 10, 1)
+    assert (c.stepsize, c.stepsize_jitter, c.init_radius) == (1.0, 0.0, 2.0)
+    assert ca.lib().nuts_output_rows(C.byref(c)) == 1000
+    assert ca.lib().nuts_output_rows(C.byref(_cfg(save_warmup=1, thin=3))) == 334 + 334
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(num_warmup=0), "num_warmup"),           # command.hpp:567-574
+    (dict(delta=1.0), "delta"),                   # arg_adapt.hpp:23-25
+    (dict(stepsize_jitter=1.5), "stepsize_jitter"),
+    (dict(max_depth=0), "max_depth"),
+    (dict(thin=0), "thin"),
+    (dict(stepsize=-1.0), "stepsize"),
+])
+def test_config_validation(kw, msg):
+    c = _cfg(**kw)
+    assert ca.lib().nuts_config_validate(C.byref(c)) == 1
+    assert msg in ca.last_error()
+    assert ca.lib().nuts_config_validate(C.byref(_cfg(num_warmup=0, adapt_engaged=0))) == 0
+
+
+def test_summary_golden(golden):
+    names, d, _ = load_stan_csv(f"{golden}/bernoulli_chain_1.csv")
+    s = summarize(d.T[:, None, :])
+    gold = {"lp__": [-7.2719, 0.0365168, 0.768874, 0.303688, -8.98426, -6.97009, -6.75007, 519.29, 503.309, 1.00141],
+            "energy__": [7.78428, 0.0508815, 1.0314, 0.745859, 6.80565, 7.46758, 9.8864, 489.874, 486.438, 1.00495],
+            "theta": [0.256552, 0.00610844, 0.119654, 0.120965, 0.0802982, 0.24996, 0.47034, 361.506, 395.736,
+                      1.00186]}
+    for k, g in gold.items():
+        assert [float(f"{v:.6g}") for v in s[names.index(k)]] == g, k
+
+
+def test_summary_matches_oracle_multichain():
+    from oracle import ess as E
+    rng = np.random.default_rng(4)
+    x = np.cumsum(rng.standard_normal((3, 4, 400)), axis=2) * 0.1 + rng.standard_normal((3, 4, 400))
+    s = summarize(x)
+    for p in range(3):
+        o = E.summary(x[p].T)
+        ref = [o["mean"], o["mcse"], o["sd"], o["mad"], *o["quantiles"], o["ess_bulk"], o["ess_tail"], o["rhat"]]
+        assert np.allclose(s[p], ref, rtol=1e-9)

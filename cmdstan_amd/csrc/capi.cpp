@@ -238,6 +238,18 @@ Dataset make_dataset(const std::string& fam, const nuts_data& d, int& kind) {
   return s;
 }
 
+int num_cus() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return 256;
+    n = pr.multiProcessorCount;
+  }
+  return n;
+}
+
 // ------------------------------------------------------------- windows
 // stan::mcmc::windowed_adaptation::set_window_params + restart (see the
 // device logic in nuts_device.h); returns the initial per-chain window state.
@@ -312,7 +324,7 @@ void validate(const nuts_config& c) {
   if (c.max_depth > NUTS_MAX_DEPTH)
     throw std::invalid_argument("max_depth above " + std::to_string(NUTS_MAX_DEPTH) + " is not supported");
   if (c.metric < 0 || c.metric > 2) throw std::invalid_argument("metric must be unit_e, diag_e or dense_e");
-  if (c.metric == NUTS_METRIC_DENSE_E) throw std::invalid_argument("dense_e is not available in this build");
+
   if (c.num_chains < 1) throw std::invalid_argument("num_chains must be >= 1");
   if (c.init_radius < 0) throw std::invalid_argument("init radius must be >= 0");
 }
@@ -349,7 +361,16 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   }
   const int maxd = cfg.max_depth;
   const int nslots = maxd + 2;
-  const int NV = PV_FIXED + 3 * maxd + 2 * nslots;
+  const bool dense = cfg.metric == NUTS_METRIC_DENSE_E;
+  const int NV = PV_FIXED + 3 * maxd + 2 * nslots + (dense ? 2 * maxd : 0);
+  int64_t mtot = 0;
+  if (dense) {
+    for (int c = 0; c < C; ++c) {
+      if (D[c] > 1024) throw std::invalid_argument("dense_e supports D <= 1024");
+      moff[c] = mtot;
+      mtot += (int64_t)D[c] * D[c];
+    }
+  }
 
   hipStream_t st;
   HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -371,7 +392,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   HIPC(hipMemsetAsync(dstop.p, 0, sizeof(int), st));
   {
     std::vector<double> hm(tot, 1.0);
-    if (R.inv_metric && cfg.metric != NUTS_METRIC_UNIT_E) {
+    if (R.inv_metric && cfg.metric == NUTS_METRIC_DIAG_E) {
       for (int c = 0; c < C; ++c)
         for (int i = 0; i < D[c]; ++i) {
           const double v = R.inv_metric[(size_t)c * D[0] + i];
@@ -380,6 +401,27 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         }
     }
     minv.upload(hm, st);
+  }
+  DevBuf<double> dMb, dLb, dWb;
+  if (dense) {
+    // column-major (i, j) at [j*D + i]; the metric file holds a symmetric matrix
+    std::vector<double> hm(mtot, 0.0);
+    for (int c = 0; c < C; ++c) {
+      const int d = D[c];
+      for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+          double v = (i == j) ? 1.0 : 0.0;
+          if (R.inv_metric) {
+            v = R.inv_metric[(size_t)c * d * d + (size_t)i * d + j];
+            if (!std::isfinite(v)) throw std::invalid_argument("inverse metric must be finite");
+          }
+          hm[moff[c] + (int64_t)j * d + i] = v;
+        }
+    }
+    dMb.upload(hm, st);
+    dLb.alloc(mtot);
+    dWb.alloc(mtot);
+    HIPC(hipMemsetAsync(dWb.p, 0, mtot * sizeof(double), st));
   }
   if (R.init) {
     std::vector<double> hi(tot, 0.0);
@@ -431,6 +473,16 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   B.n_stopped = dstop.p; B.need = dneed.p;
   B.inj_u = R.inj_u ? du.p : nullptr; B.inj_z = R.inj_z ? dz.p : nullptr;
   B.n_inj_u = R.n_u; B.n_inj_z = R.n_z;
+  B.dM = dMb.p; B.dL = dLb.p; B.dW = dWb.p;
+  if (dense) {
+    DevBuf<int> nfail(1);
+    HIPC(hipMemsetAsync(nfail.p, 0, sizeof(int), st));
+    launch_dense_factor(B, nfail.p, st);
+    int nf = 0;
+    HIPC(hipMemcpyAsync(&nf, nfail.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (nf) throw std::invalid_argument("inverse metric is not positive definite");
+  }
 
   ModelDataDev md{M.d_data.p, ddsoff.p, 1.0};
 
@@ -440,12 +492,17 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   if (!fused && M.kind == MK_LOGISTIC && !glm_ok) fused = false;  // per-chain gradient kernel in lockstep
 
   DevBuf<double> work;
+  DevBuf<int> act_idx, n_act;
   GlmArgs ga;
-  const int rows_per_chunk = 1024;
+  GlmPlan gplan;
   if (!fused && glm_ok) {
-    work.alloc(glm_workspace_doubles(M.glmN, C, M.glmD, rows_per_chunk));
-    ga.X = M.d_X; ga.y = M.d_y; ga.N = M.glmN; ga.D = M.glmD; ga.C = C; ga.rows_per_chunk = rows_per_chunk;
+    act_idx.alloc(C);
+    n_act.alloc(1);
+    gplan = glm_mfma_plan(M.glmN, C, M.glmD, num_cus());
+    work.alloc(gplan.work_doubles);
+    ga.X = M.d_X; ga.y = M.d_y; ga.N = M.glmN; ga.D = M.glmD; ga.C = C; ga.rows_per_chunk = gplan.rows_per_chunk;
     ga.s2inv = M.glm_s2inv; ga.Q = Q.p; ga.ldq = Dp[0]; ga.G = G.p; ga.V = V.p; ga.work = work.p;
+    ga.idx = act_idx.p; ga.n_active = n_act.p;
   }
 
   RunResult res;
@@ -471,7 +528,10 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
           launch_advance_external(cfg.metric, P, B, st);
           const bool t = time_grad && k == check_every / 2;
           if (t) HIPC(hipEventRecord(ev0, st));
-          if (glm_ok) { if (glm_grad_f64(ga, st)) throw std::runtime_error("GLM kernel: unsupported D"); }
+          if (glm_ok) {
+            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, st);
+            if (glm_grad_mfma_f64(ga, gplan, st)) throw std::runtime_error("GLM kernel: unsupported D");
+          }
           else launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
           if (t) { HIPC(hipEventRecord(ev1, st)); timed = true; }
           res.leaf_steps++;
@@ -514,7 +574,14 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
 
   res.st.resize(C);
   HIPC(hipMemcpyAsync(res.st.data(), dst.p, C * sizeof(ChainState), hipMemcpyDeviceToHost, st));
-  {
+  if (dense) {
+    std::vector<double> hm(mtot);
+    HIPC(hipMemcpyAsync(hm.data(), dMb.p, mtot * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    for (int c = 0; c < C; ++c)
+      for (int i = 0; i < D[c]; ++i)
+        for (int j = 0; j < D[c]; ++j) res.minv.push_back(hm[moff[c] + (int64_t)j * D[c] + i]);
+  } else {
     std::vector<double> hm(tot);
     HIPC(hipMemcpyAsync(hm.data(), minv.p, tot * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
@@ -694,10 +761,10 @@ int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points
     dD.upload(Dv); doff.upload(off); ddso.upload(dso); Q.upload(hq);
     ModelDataDev md{M.d_data.p, ddso.p, jacobian ? 1.0 : 0.0};
     if (M.glm && dataset == 0) {
-      const int rpc = 1024;
-      DevBuf<double> work(glm_workspace_doubles(M.glmN, C, D, rpc));
-      GlmArgs ga{M.d_X, M.d_y, M.glmN, D, C, rpc, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
-      if (glm_grad_f64(ga, 0)) throw std::runtime_error("GLM kernel: unsupported D");
+      const GlmPlan gp = glm_mfma_plan(M.glmN, C, D, num_cus());
+      DevBuf<double> work(gp.work_doubles);
+      GlmArgs ga{M.d_X, M.d_y, M.glmN, D, C, gp.rows_per_chunk, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
+      if (glm_grad_mfma_f64(ga, gp, 0)) throw std::runtime_error("GLM kernel: unsupported D");
       HIPC(hipDeviceSynchronize());
     } else {
       launch_model_grad(M.kind, C, dD.p, doff.p, Q.p, G.p, V.p, nullptr, md, 0);

@@ -1,0 +1,240 @@
+// Batched logistic-regression gradient on FP64 matrix cores (gfx950).
+//
+// For a tile of 64 observations and 64 chains (4 waves x 16 chains):
+//   phase 1  Eta = X_tile · Qᵀ            v_mfma_f64_16x16x4: M = rows, N = chains, K = D
+//   pointwise R = sigmoid(Eta) - y, lp += y·Eta - log1p(e^Eta)   (in registers)
+//   phase 2  Gᵀ += Rᵀ · X_tile             M = chains, N = D, K = rows
+// The 16x16x4 f64 result layout puts element (4i + l/16, l%16) in register i of
+// lane l.  Phase 2 walks the rows of the tile in the order (block rb, register
+// i, lane group l/16), so the A operand of each phase-2 step is exactly register
+// i of the phase-1 accumulator of block rb: R never leaves the registers.  The
+// B operand of phase 2 reads X in that same row order from the LDS tile that
+// phase 1 used.
+//
+// Work split: blocks own (row chunk, 64-chain group); the linear block id is
+// remapped so that the chain groups sharing one row chunk run on the same XCD
+// and share its L2.  Chunk partials go to a workspace reduced in fixed order
+// (deterministic, no atomics).
+#include "device_common.h"
+#include "launch.h"
+
+namespace nutsgpu {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+#define GM_TN 64            // rows per tile
+#define GM_TC 64            // chains per block
+#define GM_KMAX 128         // D capacity (K of phase 1, N of phase 2)
+#define GM_XS (GM_KMAX + 2) // LDS row stride in doubles (== 2 mod 32: conflict-free phase-1 reads)
+
+__global__ void __launch_bounds__(256, 2)
+glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, int D, int rows_per_chunk,
+             int nchunks, int ngroups, const double* __restrict__ Q, int64_t ldq, int C, const int* __restrict__ idx,
+             const int* __restrict__ n_active, double* __restrict__ Wg, double* __restrict__ Wlp) {
+  __shared__ double sX[GM_TN * GM_XS];
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  const int l = tid & 63;
+  const int lr = l & 15;   // m/n index inside a 16-wide tile
+  const int lk = l >> 4;   // k index inside a 4-deep step
+
+  // XCD-aware mapping: consecutive linear ids go to consecutive XCDs.
+  const int lin = blockIdx.x;
+  const int xcd = lin & 7;
+  const int j = lin >> 3;
+  const int chunk = xcd + 8 * (j / ngroups);
+  const int group = j % ngroups;
+  // chains are processed in compacted order: position p < n_act maps to chain idx[p]
+  const int n_act = idx ? *n_active : C;
+  if (chunk >= nchunks || group * GM_TC >= n_act) return;
+
+  const int c0 = group * GM_TC + 16 * w;  // this wave's 16 positions
+  const int KS = (D + 3) >> 2;            // phase-1 k-steps
+  const int DT = (D + 15) >> 4;           // phase-2 output tiles
+
+  // phase-1 B operand for all k-steps: Q[c0 + lr][4 ks + lk]
+  double qreg[GM_KMAX / 4];
+  {
+    const int pos = c0 + lr;
+    const int c = pos < n_act ? (idx ? idx[pos] : pos) : -1;
+#pragma unroll
+    for (int ks = 0; ks < GM_KMAX / 4; ++ks) {
+      const int k = 4 * ks + lk;
+      qreg[ks] = (ks < KS && c >= 0 && k < D) ? Q[(int64_t)c * ldq + k] : 0.0;
+    }
+  }
+  d4 gacc[GM_KMAX / 16];
+#pragma unroll
+  for (int t = 0; t < GM_KMAX / 16; ++t) gacc[t] = d4{0, 0, 0, 0};
+  double lp = 0.0;
+
+  const int n_begin = chunk * rows_per_chunk;
+  const int n_end = min(N, n_begin + rows_per_chunk);
+  const int ncols = DT * 16;
+
+  for (int nb = n_begin; nb < n_end; nb += GM_TN) {
+    __syncthreads();
+    for (int i = tid; i < GM_TN * ncols; i += 256) {
+      const int r = i / ncols, k = i - r * ncols;
+      const int n = nb + r;
+      sX[r * GM_XS + k] = (n < n_end && k < D) ? X[(int64_t)n * D + k] : 0.0;
+    }
+    __syncthreads();
+
+    d4 eta[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) eta[rb] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < GM_KMAX / 4; ++ks) {
+      if (ks < KS) {
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const double a = sX[(16 * rb + lr) * GM_XS + 4 * ks + lk];
+          eta[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, qreg[ks], eta[rb], 0, 0, 0);
+        }
+      }
+    }
+    // R = sigmoid(eta) - y in place; rows past the chunk contribute nothing
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int n = nb + 16 * rb + 4 * i + lk;
+        const double e = eta[rb][i];
+        double r = 0.0;
+        if (n < n_end) {
+          const double yy = y[n];
+          lp += yy * e - log1p_exp(e);
+          r = inv_logit(e) - yy;
+        }
+        eta[rb][i] = r;
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double a = eta[rb][i];
+        const double* xrow = sX + (16 * rb + 4 * i + lk) * GM_XS + lr;
+#pragma unroll
+        for (int t = 0; t < GM_KMAX / 16; ++t) {
+          if (t < DT) gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xrow[16 * t], gacc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+
+  // gacc[t] register i of lane l = G[chain c0 + 4i + lk][d 16t + lr]
+#pragma unroll
+  for (int t = 0; t < GM_KMAX / 16; ++t) {
+    if (t < DT) {
+      const int d = 16 * t + lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int pos = c0 + 4 * i + lk;
+        if (pos < n_act && d < D) Wg[((int64_t)chunk * C + pos) * D + d] = gacc[t][i];
+      }
+    }
+  }
+  // lp: lanes l, l+16, l+32, l+48 hold the same chain
+  lp += __shfl_xor(lp, 16, 64);
+  lp += __shfl_xor(lp, 32, 64);
+  if (lk == 0 && c0 + lr < n_act) Wlp[(int64_t)chunk * C + c0 + lr] = lp;
+}
+
+__global__ void glm_mfma_reduce(const double* __restrict__ Wg, const double* __restrict__ Wlp, int nchunks, int C,
+                                int D, double s2inv, const double* __restrict__ Q, int64_t ldq,
+                                const int* __restrict__ idx, const int* __restrict__ n_active,
+                                double* __restrict__ G, double* __restrict__ V) {
+  const int pos = blockIdx.x;
+  const int n_act = idx ? *n_active : C;
+  if (pos >= n_act) return;
+  const int c = idx ? idx[pos] : pos;
+  const double* q = Q + (int64_t)c * ldq;
+  double* g = G + (int64_t)c * ldq;
+  for (int k = threadIdx.x; k < D; k += blockDim.x) {
+    double s = 0.0;
+    for (int ch = 0; ch < nchunks; ++ch) s += Wg[((int64_t)ch * C + pos) * D + k];
+    g[k] = s + s2inv * q[k];
+  }
+  if (threadIdx.x < 64) {
+    double lpv = 0.0, pr = 0.0;
+    for (int ch = threadIdx.x; ch < nchunks; ch += 64) lpv += Wlp[(int64_t)ch * C + pos];
+    for (int k = threadIdx.x; k < D; k += 64) pr += q[k] * q[k];
+    lpv = wave_sum(lpv);
+    pr = wave_sum(pr);
+    if (threadIdx.x == 0) V[c] = -(lpv - 0.5 * s2inv * pr);
+  }
+}
+
+// Active-chain list for the lockstep gradient: idx[0..n) = chains with need[c]
+// set, in chain order (single-block scan, deterministic); clears need[].
+__global__ void __launch_bounds__(1024) compact_active(int* __restrict__ need, int C, int* __restrict__ idx,
+                                                       int* __restrict__ n_active) {
+  __shared__ int wsum[16];
+  __shared__ int base;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) base = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < C; c0 += 1024) {
+    const int c = c0 + t;
+    const int f = (c < C && need[c]) ? 1 : 0;
+    if (c < C) need[c] = 0;
+    int x = f;  // inclusive wave scan
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int off = base;
+    for (int k = 0; k < wv; ++k) off += wsum[k];
+    if (f) idx[off + x - 1] = c;
+    __syncthreads();
+    if (t == 1023) {
+      int tot = 0;
+      for (int k = 0; k < 16; ++k) tot += wsum[k];
+      base += tot;
+    }
+    __syncthreads();
+  }
+  if (t == 0) *n_active = base;
+}
+
+void launch_compact_active(int* need, int C, int* idx, int* n_active, hipStream_t st) {
+  hipLaunchKernelGGL(compact_active, dim3(1), dim3(1024), 0, st, need, C, idx, n_active);
+}
+
+GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus) {
+  GlmPlan p;
+  p.ngroups = (C + GM_TC - 1) / GM_TC;
+  // about two resident blocks per CU, chunk count a multiple of the 8 XCDs
+  int target = 2 * (num_cus > 0 ? num_cus : 256);
+  int nch = (target + p.ngroups - 1) / p.ngroups;
+  nch = ((nch + 7) / 8) * 8;
+  const int max_chunks = ((N + GM_TN - 1) / GM_TN + 7) / 8 * 8;
+  if (nch > max_chunks) nch = max_chunks;
+  if (nch < 8) nch = 8;
+  int rpc = (N + nch - 1) / nch;
+  rpc = ((rpc + GM_TN - 1) / GM_TN) * GM_TN;
+  p.rows_per_chunk = rpc;
+  p.nchunks = nch;
+  p.work_doubles = (size_t)nch * C * D + (size_t)nch * C;
+  (void)D;
+  return p;
+}
+
+int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
+  if (a.D > GM_KMAX || a.D < 1) return 1;
+  double* Wg = a.work;
+  double* Wlp = a.work + (size_t)p.nchunks * a.C * a.D;
+  const int nblocks = (p.nchunks / 8) * p.ngroups * 8;
+  hipLaunchKernelGGL(glm_mfma_f64, dim3(nblocks), dim3(256), 0, st, a.X, a.y, a.N, a.D, p.rows_per_chunk, p.nchunks,
+                     p.ngroups, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp);
+  hipLaunchKernelGGL(glm_mfma_reduce, dim3(a.C), dim3(128), 0, st, Wg, Wlp, p.nchunks, a.C, a.D, a.s2inv, a.Q, a.ldq,
+                     a.idx, a.n_active, a.G, a.V);
+  return 0;
+}
+
+}  // namespace nutsgpu

@@ -27,6 +27,8 @@ struct ModelDataDev {
 // Advance every chain to its next gradient request (lockstep) or through up
 // to P.fused_budget gradients computed in place by the model (fused).
 void launch_advance_external(int metric, const NutsParams& P, const NutsBuffers& B, hipStream_t st);
+// Cholesky factor of every chain's dense inverse metric (B.dM -> B.dL); counts failures.
+void launch_dense_factor(const NutsBuffers& B, int* failed, hipStream_t st);
 void launch_advance_fused(int kind, int metric, const NutsParams& P, const NutsBuffers& B, const ModelDataDev& md,
                           hipStream_t st);
 
@@ -47,8 +49,19 @@ struct GlmArgs {
   const double* Q; int64_t ldq;
   double* G; double* V;
   double* work;
+  const int* idx = nullptr;       // optional compacted chain list (positions -> chains)
+  const int* n_active = nullptr;  // its length (device)
 };
 size_t glm_workspace_doubles(int N, int C, int D, int rows_per_chunk);
 int glm_grad_f64(const GlmArgs& a, hipStream_t st);
+
+// FP64 matrix-core version (the sampler's default for logistic regression).
+struct GlmPlan {
+  int ngroups = 0, nchunks = 0, rows_per_chunk = 0;
+  size_t work_doubles = 0;
+};
+GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus);
+void launch_compact_active(int* need, int C, int* idx, int* n_active, hipStream_t st);
+int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st);
 
 }  // namespace nutsgpu

@@ -18,6 +18,13 @@
 // strided over the 64 lanes and reductions are butterfly shuffles, so every
 // lane holds identical scalar state (the staged ChainState in LDS is written
 // with identical values by all lanes).
+//
+// Metrics (stan::mcmc::{unit,diag,dense}_e_metric): the "sharp" momentum
+// Minv·p enters the leapfrog and the U-turn checks.  For unit/diag it is formed
+// on the fly; for dense_e every momentum kept for a U-turn check has a stored
+// sharp companion, computed once per leaf by a matvec with the per-chain D×D
+// inverse metric, and momenta are drawn by the triangular solve Lᵀ p = z with
+// the chain's Cholesky factor Minv = L Lᵀ.
 #pragma once
 #include "device_common.h"
 #include "nuts_types.h"
@@ -29,8 +36,22 @@ struct ExternalGrad {
   __device__ double operator()(int, const double*, double*, int, int) const { return 0.0; }
 };
 
+// dense_e keeps one momentum in registers during the triangular solve:
+// D <= 64 * DENSE_RMAX
+#define DENSE_RMAX 16
+
+// Orders this lane's earlier global/LDS writes before later reads of other
+// lanes of the same wavefront (vectors are written lane-strided and then read
+// whole by every lane in matvecs and model gradients).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int METRIC, class GradFn>
 struct NutsWave {
+  static constexpr bool DENSE = METRIC == METRIC_DENSE;
   const NutsParams& P;
   const NutsBuffers& B;
   ChainState& s;
@@ -40,7 +61,10 @@ struct NutsWave {
   double* q;     // Z end position (dense Q row)
   double* g;     // Z end potential gradient (dense G row)
   double* blk;   // pool block of this chain
-  double* minv;
+  double* minv;  // diagonal inverse metric (unit/diag)
+  double* dM;    // dense: inverse metric, column-major D×D
+  double* dL;    // dense: lower Cholesky factor, column-major
+  double* dW;    // dense: Welford m2, column-major
 
   __device__ NutsWave(const NutsParams& P_, const NutsBuffers& B_, ChainState& s_, const GradFn& g_, int c_, int lane_)
       : P(P_), B(B_), s(s_), grad(g_), c(c_), lane(lane_) {
@@ -51,6 +75,13 @@ struct NutsWave {
     g = B.G + B.off[c];
     blk = B.pool + B.off[c] * (int64_t)P.NV;
     minv = B.minv + B.off[c];
+    if (DENSE) {
+      dM = B.dM + B.moff[c];
+      dL = B.dL + B.moff[c];
+      dW = B.dW + B.moff[c];
+    } else {
+      dM = dL = dW = nullptr;
+    }
   }
 
   __device__ double* vecp(int id) const { return blk + (int64_t)id * Dp; }
@@ -59,14 +90,45 @@ struct NutsWave {
   __device__ double* Lrho(int k) const { return vecp(PV_FIXED + 2 * P.maxd + k); }
   __device__ double* Sq(int k) const { return vecp(PV_FIXED + 3 * P.maxd + k); }
   __device__ double* Sg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + P.nslots + k); }
+  __device__ double* LSbeg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + 2 * P.nslots + k); }
+  __device__ double* LSend(int k) const { return vecp(PV_FIXED + 4 * P.maxd + 2 * P.nslots + k); }
 
   // ------------------------------------------------------------- metric
   __device__ double mi(int i) const { return METRIC == METRIC_UNIT ? 1.0 : minv[i]; }
-  __device__ double sharp(const double* x, int i) const { return METRIC == METRIC_UNIT ? x[i] : minv[i] * x[i]; }
-
-  __device__ double kinetic(const double* p) const {
+  // sharp momentum element: x = momentum, xs = its stored companion (dense)
+  __device__ double sh(const double* x, const double* xs, int i) const {
+    if (DENSE) return xs[i];
+    return METRIC == METRIC_UNIT ? x[i] : minv[i] * x[i];
+  }
+  // out = Minv x (dense), summing j = 0..D-1 in order like Eigen's row product
+  __device__ void matvec(const double* x, double* out) const {
+    wave_sync();
+    constexpr int RB = 8;
+    for (int i0 = 0; i0 < D; i0 += NUTS_WAVE * RB) {
+      double acc[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) acc[r] = 0.0;
+      for (int j = 0; j < D; ++j) {
+        const double xj = x[j];
+        const double* col = dM + (int64_t)j * D;  // column j = row j of Minv
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          const int i = i0 + lane + NUTS_WAVE * r;
+          if (i < D) acc[r] += col[i] * xj;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int i = i0 + lane + NUTS_WAVE * r;
+        if (i < D) out[i] = acc[r];
+      }
+    }
+    wave_sync();
+  }
+  // T = 1/2 p·Minv p; for dense ps must hold Minv p
+  __device__ double kinetic(const double* p, const double* ps) const {
     double a = 0;
-    for (int i = lane; i < D; i += NUTS_WAVE) a += p[i] * sharp(p, i);
+    for (int i = lane; i < D; i += NUTS_WAVE) a += p[i] * sh(p, ps, i);
     return 0.5 * wave_sum(a);
   }
 
@@ -76,23 +138,59 @@ struct NutsWave {
     if (B.inj_u) return k < (uint64_t)B.n_inj_u ? B.inj_u[(int64_t)c * B.n_inj_u + k] : 0.5;
     return stream_uniform(P.seed, uid, STREAM_UNIFORM, k);
   }
-  // diag_e_metric::sample_p / unit_e_metric::sample_p
-  __device__ void sample_p(double* p) {
+  __device__ double normal_draw(int i) const {
     if (B.inj_z) {
-      const double* z = B.inj_z + (int64_t)c * B.n_inj_z;
+      const uint64_t k = s.nctr + (uint64_t)i;
+      return k < (uint64_t)B.n_inj_z ? B.inj_z[(int64_t)c * B.n_inj_z + k] : 0.0;
+    }
+    return stream_normal(P.seed, uid, s.nctr + (uint64_t)(i >> 1), i & 1);
+  }
+  __device__ void advance_normals() { s.nctr += B.inj_z ? (uint64_t)D : (uint64_t)((D + 1) / 2); }
+
+  // {unit,diag,dense}_e_metric::sample_p; for dense also sets ps = Minv p
+  __device__ void sample_p(double* p, double* ps) {
+    if (!DENSE) {
       for (int i = lane; i < D; i += NUTS_WAVE) {
-        const uint64_t k = s.nctr + (uint64_t)i;
-        const double zi = k < (uint64_t)B.n_inj_z ? z[k] : 0.0;
-        p[i] = METRIC == METRIC_UNIT ? zi : zi / sqrt(minv[i]);
+        const double z = normal_draw(i);
+        p[i] = METRIC == METRIC_UNIT ? z : z / sqrt(minv[i]);
       }
-      s.nctr += D;
+      advance_normals();
       return;
     }
-    for (int i = lane; i < D; i += NUTS_WAVE) {
-      const double z = stream_normal(P.seed, uid, s.nctr + (uint64_t)(i >> 1), i & 1);
-      p[i] = METRIC == METRIC_UNIT ? z : z / sqrt(minv[i]);
+    // back substitution Lᵀ p = z (U = Lᵀ upper) with p held in registers:
+    // lane l owns elements l + 64 r; column i of L below the diagonal is
+    // contiguous in the column-major factor
+    double pr[DENSE_RMAX];
+#pragma unroll
+    for (int r = 0; r < DENSE_RMAX; ++r) {
+      const int i = lane + NUTS_WAVE * r;
+      pr[r] = i < D ? normal_draw(i) : 0.0;
     }
-    s.nctr += (uint64_t)((D + 1) / 2);
+    advance_normals();
+    for (int i = D - 1; i >= 0; --i) {
+      const double* Li = dL + (int64_t)i * D;
+      const int ri = i >> 6, li = i & 63;
+      double a = 0, zi = 0;
+#pragma unroll
+      for (int r = 0; r < DENSE_RMAX; ++r) {
+        const int j = lane + NUTS_WAVE * r;
+        if (j > i && j < D) a += Li[j] * pr[r];
+        if (r == ri) zi = pr[r];
+      }
+      a = wave_sum(a);
+      zi = __shfl(zi, li, NUTS_WAVE);
+      const double pi = (zi - a) / Li[i];
+#pragma unroll
+      for (int r = 0; r < DENSE_RMAX; ++r)
+        if (r == ri && lane == li) pr[r] = pi;
+    }
+#pragma unroll
+    for (int r = 0; r < DENSE_RMAX; ++r) {
+      const int i = lane + NUTS_WAVE * r;
+      if (i < D) p[i] = pr[r];
+    }
+    wave_sync();
+    matvec(p, ps);
   }
 
   // ------------------------------------------------------------- vectors
@@ -106,17 +204,25 @@ struct NutsWave {
   __device__ void leap_begin(double e) const {
     double* p = vecp(PV_P);
     const double h = 0.5 * e;
+    if (DENSE) {
+      double* t = vecp(PV_TMP);
+      for (int i = lane; i < D; i += NUTS_WAVE) p[i] = p[i] - h * g[i];
+      matvec(p, t);
+      for (int i = lane; i < D; i += NUTS_WAVE) q[i] = q[i] + e * t[i];
+      return;
+    }
     for (int i = lane; i < D; i += NUTS_WAVE) {
       const double pi = p[i] - h * g[i];
       p[i] = pi;
       q[i] = q[i] + e * (mi(i) * pi);
     }
   }
-  // end_update_p(eps/2)
+  // end_update_p(eps/2); dense: also the leaf's sharp momentum
   __device__ void leap_end(double e) const {
     double* p = vecp(PV_P);
     const double h = 0.5 * e;
     for (int i = lane; i < D; i += NUTS_WAVE) p[i] = p[i] - h * g[i];
+    if (DENSE) matvec(p, vecp(PV_PS));
   }
 
   // ------------------------------------------------------------- slots
@@ -191,8 +297,9 @@ struct NutsWave {
     copy(g, Sg(k));
     s.Vz = s.slot_V[k];
     double* p = vecp(PV_P);
-    sample_p(p);
-    s.ss_H0 = kinetic(p) + s.Vz;
+    double* ps = vecp(PV_PS);
+    sample_p(p, ps);
+    s.ss_H0 = kinetic(p, ps) + s.Vz;
     leap_begin(s.nom_eps);
     return request(PH_SS_PROBE);
   }
@@ -205,7 +312,7 @@ struct NutsWave {
   __device__ bool ss_probe() {
     s.Vz = B.V[c];
     leap_end(s.nom_eps);
-    double h = kinetic(vecp(PV_P)) + s.Vz;
+    double h = kinetic(vecp(PV_P), vecp(PV_PS)) + s.Vz;
     if (isnan(h)) h = __builtin_inf();
     const double dH = s.ss_H0 - h;
     const double l08 = log(0.8);
@@ -242,10 +349,11 @@ struct NutsWave {
     const uint32_t all = P.nslots >= 32 ? 0xFFFFFFFFu : ((1u << P.nslots) - 1u);
     s.free_mask = all & ~(1u << k);
     double* p = vecp(PV_P);
+    double* ps = vecp(PV_PS);
     copy(q, Sq(k));
     copy(g, Sg(k));
     s.Vz = s.slot_V[k];
-    sample_p(p);
+    sample_p(p, ps);
     double* oq = vecp(PV_OQ); double* op = vecp(PV_OP); double* og = vecp(PV_OG);
     double* pfb = vecp(PV_PFB); double* pbf = vecp(PV_PBF); double* rho = vecp(PV_RHO);
     for (int i = lane; i < D; i += NUTS_WAVE) {
@@ -253,10 +361,14 @@ struct NutsWave {
       oq[i] = q[i]; op[i] = pi; og[i] = g[i];
       pfb[i] = pi; pbf[i] = pi; rho[i] = pi;
     }
+    if (DENSE) {
+      double* ops = vecp(PV_OPS); double* psfb = vecp(PV_PSFB); double* psbf = vecp(PV_PSBF);
+      for (int i = lane; i < D; i += NUTS_WAVE) { const double v = ps[i]; ops[i] = v; psfb[i] = v; psbf[i] = v; }
+    }
     s.Vo = s.Vz;
     s.z_is_fwd = 1;
     s.log_w = 0;
-    s.H0 = kinetic(p) + s.Vz;
+    s.H0 = kinetic(p, ps) + s.Vz;
     s.slot_H[k] = s.H0;
     s.n_leapfrog = 0;
     s.sum_metro = 0;
@@ -271,12 +383,18 @@ struct NutsWave {
       swap(q, vecp(PV_OQ));
       swap(vecp(PV_P), vecp(PV_OP));
       swap(g, vecp(PV_OG));
+      if (DENSE) swap(vecp(PV_PS), vecp(PV_OPS));
       const double t = s.Vz; s.Vz = s.Vo; s.Vo = t;
       s.z_is_fwd = fwd;
     }
     // p_bck_fwd = p_fwd_bck (forward) / p_fwd_bck = p_bck_fwd (backward)
-    if (fwd) copy(vecp(PV_PBF), vecp(PV_PFB));
-    else copy(vecp(PV_PFB), vecp(PV_PBF));
+    if (fwd) {
+      copy(vecp(PV_PBF), vecp(PV_PFB));
+      if (DENSE) copy(vecp(PV_PSBF), vecp(PV_PSFB));
+    } else {
+      copy(vecp(PV_PFB), vecp(PV_PBF));
+      if (DENSE) copy(vecp(PV_PSFB), vecp(PV_PSBF));
+    }
     s.dir = fwd ? 1 : -1;
     s.leaf_idx = 0;
     leap_begin(s.dir * s.eps);
@@ -289,15 +407,19 @@ struct NutsWave {
     s.Vz = B.V[c];
     leap_end(e);
     double* p = vecp(PV_P);
+    double* ps = vecp(PV_PS);
     s.n_leapfrog++;
-    double h = kinetic(p) + s.Vz;
+    double h = kinetic(p, ps) + s.Vz;
     if (isnan(h)) h = __builtin_inf();
     if ((h - s.H0) > P.max_deltaH) s.divergent = 1;
     const double lw = s.H0 - h;
     if (s.H0 - h > 0) s.sum_metro += 1;
     else s.sum_metro += exp(s.H0 - h);
     s.leaf_h = h;
-    if (s.leaf_idx == 0) copy(s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF), p);
+    if (s.leaf_idx == 0) {
+      copy(s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF), p);
+      if (DENSE) copy(s.z_is_fwd ? vecp(PV_PSFB) : vecp(PV_PSBF), ps);
+    }
     if (s.divergent) { trans_end(); return false; }
 
     double* srho = vecp(PV_SRHO);
@@ -316,15 +438,18 @@ struct NutsWave {
       const double* le = Lend(k);
       const double* lr = Lrho(k);
       const double* sb = k == 0 ? p : Lbeg(k - 1);
+      const double* lbs = DENSE ? LSbeg(k) : nullptr;
+      const double* les = DENSE ? LSend(k) : nullptr;
+      const double* sbs = DENSE ? (k == 0 ? ps : LSbeg(k - 1)) : nullptr;
       double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
       for (int i = lane; i < D; i += NUTS_WAVE) {
         const double rs = lr[i] + srho[i];
         const double r2 = lr[i] + sb[i];
         const double r3 = srho[i] + le[i];
-        const double slb = sharp(lb, i), sp = sharp(p, i);
+        const double slb = sh(lb, lbs, i), sp = sh(p, ps, i);
         a1 += slb * rs; b1 += sp * rs;
-        a2 += slb * r2; b2 += sharp(sb, i) * r2;
-        a3 += sharp(le, i) * r3; b3 += sp * r3;
+        a2 += slb * r2; b2 += sh(sb, sbs, i) * r2;
+        a3 += sh(le, les, i) * r3; b3 += sp * r3;
         srho[i] = rs;
       }
       a1 = wave_sum(a1); b1 = wave_sum(b1); a2 = wave_sum(a2);
@@ -339,6 +464,10 @@ struct NutsWave {
     copy(Lbeg(k), k == 0 ? p : Lbeg(k - 1));
     copy(Lend(k), p);
     copy(Lrho(k), srho);
+    if (DENSE) {
+      copy(LSbeg(k), k == 0 ? ps : LSbeg(k - 1));
+      copy(LSend(k), ps);
+    }
     s.L_logw[k] = s.S_logw;
     s.L_prop[k] = s.s_prop == PROP_LEAF ? store_leaf_slot() : s.s_prop;
     s.leaf_idx++;
@@ -367,8 +496,14 @@ struct NutsWave {
     const double* pbf = vecp(PV_PBF);
     const double* srho = vecp(PV_SRHO);
     double* rho = vecp(PV_RHO);
+    const double* ps = DENSE ? vecp(PV_PS) : nullptr;
+    const double* ops = DENSE ? vecp(PV_OPS) : nullptr;
+    const double* psfb = DENSE ? vecp(PV_PSFB) : nullptr;
+    const double* psbf = DENSE ? vecp(PV_PSBF) : nullptr;
     const double* bb = fwd ? op : p;   // p_bck_bck
     const double* ff = fwd ? p : op;   // p_fwd_fwd
+    const double* bbs = fwd ? ops : ps;
+    const double* ffs = fwd ? ps : ops;
     double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
     for (int i = lane; i < D; i += NUTS_WAVE) {
       const double old = rho[i], nw = srho[i];
@@ -376,10 +511,10 @@ struct NutsWave {
       const double rb = fwd ? old : nw, rf = fwd ? nw : old;
       const double r2 = rb + pfb[i];
       const double r3 = rf + pbf[i];
-      const double sbb = sharp(bb, i), sff = sharp(ff, i);
+      const double sbb = sh(bb, bbs, i), sff = sh(ff, ffs, i);
       a1 += sbb * rn; b1 += sff * rn;
-      a2 += sbb * r2; b2 += sharp(pfb, i) * r2;
-      a3 += sharp(pbf, i) * r3; b3 += sff * r3;
+      a2 += sbb * r2; b2 += sh(pfb, psfb, i) * r2;
+      a3 += sh(pbf, psbf, i) * r3; b3 += sff * r3;
       rho[i] = rn;
     }
     a1 = wave_sum(a1); b1 = wave_sum(b1); a2 = wave_sum(a2);
@@ -404,41 +539,75 @@ struct NutsWave {
       if (boundary >= P.w_num_warmup - P.w_term_buffer) s.w_next = P.w_num_warmup - P.w_term_buffer - 1;
     }
   }
-  // var_adaptation::learn_variance; returns true at a window end
-  __device__ bool learn_variance(const double* qs) {
+  // var_adaptation::learn_variance / covar_adaptation::learn_covariance;
+  // returns true at a window end
+  __device__ bool learn_metric(const double* qs) {
     double* wm = vecp(PV_WM);
     double* wm2 = vecp(PV_WM2);
     if (in_window()) {
       s.welford_n++;
       const double n = (double)s.welford_n;
-      for (int i = lane; i < D; i += NUTS_WAVE) {
-        const double delta = qs[i] - wm[i];
-        const double m = wm[i] + delta / n;
-        wm[i] = m;
-        wm2[i] = wm2[i] + (qs[i] - m) * delta;
+      if (!DENSE) {
+        for (int i = lane; i < D; i += NUTS_WAVE) {
+          const double delta = qs[i] - wm[i];
+          const double m = wm[i] + delta / n;
+          wm[i] = m;
+          wm2[i] = wm2[i] + (qs[i] - m) * delta;
+        }
+      } else {
+        // welford_covar_estimator: delta = q - m; m += delta/n; m2 += (q - m) deltaᵀ
+        double* delta = vecp(PV_TMP);
+        for (int i = lane; i < D; i += NUTS_WAVE) {
+          const double dl = qs[i] - wm[i];
+          delta[i] = dl;
+          wm[i] = wm[i] + dl / n;
+          wm2[i] = qs[i] - wm[i];  // (q - m_new), kept in the diag accumulator slot
+        }
+        wave_sync();
+        for (int j = 0; j < D; ++j) {
+          const double dj = delta[j];
+          double* col = dW + (int64_t)j * D;
+          for (int i = lane; i < D; i += NUTS_WAVE) col[i] = col[i] + wm2[i] * dj;
+        }
       }
     }
     if (end_window()) {
       next_window();
       const double n = (double)s.welford_n;
       int ok = 1;
-      for (int i = lane; i < D; i += NUTS_WAVE) {
-        const double var = s.welford_n > 1 ? wm2[i] / (n - 1.0) : minv[i];
-        const double v = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0));
-        minv[i] = v;
-        ok &= isfinite(v) ? 1 : 0;
-        wm[i] = 0; wm2[i] = 0;
+      if (!DENSE) {
+        for (int i = lane; i < D; i += NUTS_WAVE) {
+          const double var = s.welford_n > 1 ? wm2[i] / (n - 1.0) : minv[i];
+          const double v = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0));
+          minv[i] = v;
+          ok &= isfinite(v) ? 1 : 0;
+          wm[i] = 0; wm2[i] = 0;
+        }
+      } else {
+        for (int j = 0; j < D; ++j) {
+          double* Mj = dM + (int64_t)j * D;
+          double* Wj = dW + (int64_t)j * D;
+          for (int i = lane; i < D; i += NUTS_WAVE) {
+            const double cov = s.welford_n > 1 ? Wj[i] / (n - 1.0) : Mj[i];
+            const double v = (n / (n + 5.0)) * cov + (i == j ? 1e-3 * (5.0 / (n + 5.0)) : 0.0);
+            Mj[i] = v;
+            ok &= isfinite(v) ? 1 : 0;
+            Wj[i] = 0;
+          }
+        }
+        for (int i = lane; i < D; i += NUTS_WAVE) { wm[i] = 0; wm2[i] = 0; }
       }
       s.welford_n = 0;
       s.w_counter++;
       if (!wave_and(ok)) { fail(FAIL_METRIC_OVERFLOW); return false; }
+      if (DENSE && !dense_factor(dM, dL, D, lane)) { fail(FAIL_METRIC_OVERFLOW); return false; }
       return true;
     }
     s.w_counter++;
     return false;
   }
 
-  // end of base_nuts::transition + adapt_diag_e_nuts::transition + writer
+  // end of base_nuts::transition + adapt_*_e_nuts::transition + writer
   __device__ void trans_end() {
     const double accept = s.sum_metro / (double)s.n_leapfrog;
     const int k = s.sample_slot;
@@ -470,7 +639,7 @@ struct NutsWave {
       s.sa_xbar = (1.0 - x_eta) * s.sa_xbar + x_eta * x;
       s.nom_eps = exp(x);
       if (METRIC != METRIC_UNIT && P.metric_adapt) {
-        const bool update = learn_variance(Sq(k));
+        const bool update = learn_metric(Sq(k));
         if (s.phase == PH_FAILED) return;
         if (update) { s.ss_return = SS_RET_WINDOW; s.phase = PH_SS_START; return; }
       }
@@ -509,12 +678,49 @@ struct NutsWave {
       const bool req = dispatch();
       if (!req) continue;
       if (!GradFn::fused) { if (lane == 0) B.need[c] = 1; return; }
+      wave_sync();
       const double V = grad(c, q, g, D, lane);
+      wave_sync();
       if (lane == 0) B.V[c] = V;
       if (--budget == 0) return;
     }
   }
+
+  // Lower Cholesky factor of the column-major SPD matrix A into L (column-major),
+  // left-looking, one column per step with the wave splitting the rows.
+  // Returns false if a pivot is not positive.
+  static __device__ bool dense_factor(const double* A, double* L, int D, int lane) {
+    for (int j = 0; j < D; ++j) {
+      // pivot: A[j][j] - sum_k L[j][k]^2 (row j of L = element j of columns k < j)
+      double sq = 0;
+      for (int k = lane; k < j; k += NUTS_WAVE) { const double v = L[(int64_t)k * D + j]; sq += v * v; }
+      sq = wave_sum(sq);
+      const double piv = A[(int64_t)j * D + j] - sq;
+      if (!(piv > 0)) return false;
+      const double d = sqrt(piv);
+      double* Lj = L + (int64_t)j * D;
+      for (int i = j + 1 + lane; i < D; i += NUTS_WAVE) {
+        double v = A[(int64_t)j * D + i];  // A[i][j] (lower triangle)
+        for (int k = 0; k < j; ++k) v -= L[(int64_t)k * D + i] * L[(int64_t)k * D + j];
+        Lj[i] = v / d;
+      }
+      for (int i = lane; i < j; i += NUTS_WAVE) Lj[i] = 0.0;
+      if (lane == 0) Lj[j] = d;
+      wave_sync();
+    }
+    return true;
+  }
 };
+
+// Initial factorisation of every chain's dense inverse metric (one wave per chain).
+__global__ void __launch_bounds__(256) dense_factor_kernel(NutsBuffers B, int* failed) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= B.C) return;
+  const int D = B.D[c];
+  const bool ok = NutsWave<METRIC_DENSE, ExternalGrad>::dense_factor(B.dM + B.moff[c], B.dL + B.moff[c], D, lane);
+  if (!ok && lane == 0) atomicAdd(failed, 1);
+}
 
 // Stage ChainState in LDS, run, write back.  One wave per chain; 4 waves per block.
 template <int METRIC, class GradFn>

@@ -48,13 +48,20 @@ static int blocks_for(int C) { return (C + 3) / 4; }
 void launch_advance_external(int metric, const NutsParams& P, const NutsBuffers& B, hipStream_t st) {
   const dim3 grid(blocks_for(B.C)), blk(256);
   if (metric == METRIC_UNIT) hipLaunchKernelGGL(advance_external_kernel<METRIC_UNIT>, grid, blk, 0, st, P, B);
+  else if (metric == METRIC_DENSE) hipLaunchKernelGGL(advance_external_kernel<METRIC_DENSE>, grid, blk, 0, st, P, B);
   else hipLaunchKernelGGL(advance_external_kernel<METRIC_DIAG>, grid, blk, 0, st, P, B);
+}
+
+void launch_dense_factor(const NutsBuffers& B, int* failed, hipStream_t st) {
+  hipLaunchKernelGGL(dense_factor_kernel, dim3(blocks_for(B.C)), dim3(256), 0, st, B, failed);
 }
 
 template <class F>
 static void fused_for_metric(int metric, const NutsParams& P, const NutsBuffers& B, const F& f, hipStream_t st) {
   const dim3 grid(blocks_for(B.C)), blk(256);
   if (metric == METRIC_UNIT) hipLaunchKernelGGL((advance_fused_kernel<METRIC_UNIT, F>), grid, blk, 0, st, P, B, f);
+  else if (metric == METRIC_DENSE)
+    hipLaunchKernelGGL((advance_fused_kernel<METRIC_DENSE, F>), grid, blk, 0, st, P, B, f);
   else hipLaunchKernelGGL((advance_fused_kernel<METRIC_DIAG, F>), grid, blk, 0, st, P, B, f);
 }
 

@@ -101,12 +101,19 @@ struct NutsBuffers {
   const double* inj_u;      // injected uniforms [C][n_inj_u] (replay) or null
   const double* inj_z;      // injected normals [C][n_inj_z] or null
   int64_t n_inj_u, n_inj_z;
+  // dense_e: per-chain D*D blocks at moff[c], all column-major (element (i, j) at [j*D + i])
+  double* dM;               // inverse metric
+  double* dL;               // its lower Cholesky factor
+  double* dW;               // Welford covariance accumulator m2
 };
 
 // Vector ids inside a chain's pool block.
 enum PoolVec : int {
   PV_P = 0, PV_OQ, PV_OP, PV_OG, PV_PFB, PV_PBF, PV_RHO, PV_SRHO, PV_WM, PV_WM2,
-  PV_FIXED  // then L_beg[maxd], L_end[maxd], L_rho[maxd], slot_q[nslots], slot_g[nslots]
+  // dense_e companions: Minv times the momentum of the same role, and scratch
+  PV_PS, PV_OPS, PV_PSFB, PV_PSBF, PV_TMP,
+  PV_FIXED  // then L_beg[maxd], L_end[maxd], L_rho[maxd], slot_q[nslots], slot_g[nslots],
+            // and for dense_e LS_beg[maxd], LS_end[maxd]
 };
 
 }  // namespace nutsgpu

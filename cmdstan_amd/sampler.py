@@ -148,7 +148,8 @@ class Model:
         total = sum(rows * c for c in cols)
         draws = np.zeros(total)
         step = np.zeros(C_tot)
-        msz = sum(self.dims[m] for m in ds_of_chain)
+        dense = cfg.metric == 2
+        msz = sum(self.dims[m] ** 2 if dense else self.dims[m] for m in ds_of_chain)
         imet = np.zeros(msz)
         nlf = np.zeros(C_tot, dtype=np.int64)
         nge = np.zeros(C_tot, dtype=np.int64)
@@ -166,8 +167,12 @@ class Model:
             per_chain.append(draws[o:o + n].reshape(rows, cols[k]))
             o += n
             D = self.dims[ds_of_chain[k]]
-            metrics.append(imet[mo:mo + D])
-            mo += D
+            if dense:
+                metrics.append(imet[mo:mo + D * D].reshape(D, D))
+                mo += D * D
+            else:
+                metrics.append(imet[mo:mo + D])
+                mo += D
         fit = Fit(draws=per_chain, stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
                   status=status, warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds,
                   leaf_steps=out.leaf_steps, grad_seconds_per_launch=out.grad_seconds_per_launch,
@@ -186,8 +191,10 @@ class Model:
         if u.shape[0] != nch or z.shape[0] != nch:
             raise ValueError("one row of draws per chain")
         m = METRICS[metric] if isinstance(metric, str) else metric
-        im = _f64(np.broadcast_to(np.ones(D) if inv_metric is None else np.asarray(inv_metric, dtype=np.float64),
-                                  (nch, D)))
+        msz = D * D if m == 2 else D
+        base = (np.eye(D).ravel() if m == 2 else np.ones(D)) if inv_metric is None else \
+            np.asarray(inv_metric, dtype=np.float64).ravel()
+        im = _f64(np.broadcast_to(base, (nch, msz)))
         out_q = np.zeros((nch, D))
         info = np.zeros((nch, 9))
         rc = lib().nuts_transition_replay(self.h, dataset, nch, m, stepsize, jitter, max_depth, _dp(im), _dp(q0),

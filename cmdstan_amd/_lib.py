@@ -20,7 +20,7 @@ class nuts_config(C.Structure):
         ("t0", C.c_double), ("init_buffer", C.c_int), ("term_buffer", C.c_int), ("window", C.c_int),
         ("max_depth", C.c_int), ("metric", C.c_int), ("stepsize", C.c_double), ("stepsize_jitter", C.c_double),
         ("num_chains", C.c_int), ("id", C.c_int), ("seed", C.c_uint64), ("init_radius", C.c_double),
-        ("fused", C.c_int),
+        ("fused", C.c_int), ("precision", C.c_int),
     ]
 
 
@@ -46,6 +46,7 @@ SIGNATURES = {
     "nuts_model_param_name": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "nuts_model_unconstrained_name": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "nuts_log_prob_grad": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_int, C.c_int, DP, DP]),
+    "nuts_log_prob_grad_precision": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_int, C.c_int, C.c_int, DP, DP]),
     "nuts_config_default": (None, [C.POINTER(nuts_config)]),
     "nuts_config_validate": (C.c_int, [C.POINTER(nuts_config)]),
     "nuts_output_rows": (C.c_int, [C.POINTER(nuts_config)]),

@@ -124,7 +124,22 @@ struct ModelImpl {
   double glm_s2inv = 0;
   const double* d_X = nullptr;
   const double* d_y = nullptr;
+  DevBuf<unsigned char> xlow;  // X converted to FP32 or BF16 (lazily)
+  int xlow_prec = 0;
+  const void* low_x(int precision, hipStream_t st) {
+    if (xlow_prec != precision) {
+      const int64_t n = (int64_t)glmN * glmD;
+      xlow.alloc((size_t)n * (precision == 1 ? 4 : 2));
+      glm_convert_x(d_X, xlow.p, n, precision, st);
+      HIPC(hipStreamSynchronize(st));
+      xlow_prec = precision;
+    }
+    return xlow.p;
+  }
 };
+
+// Register layout of the 16x16 f32 MFMA result (0: row 4*(l/16)+i, 1: row 4i + l/16).
+constexpr int kF32Layout = 0;
 
 std::vector<std::string> idx_names(const std::string& base, int n) {
   std::vector<std::string> r;
@@ -327,6 +342,7 @@ void validate(const nuts_config& c) {
 
   if (c.num_chains < 1) throw std::invalid_argument("num_chains must be >= 1");
   if (c.init_radius < 0) throw std::invalid_argument("init radius must be >= 0");
+  if (c.precision < 0 || c.precision > 2) throw std::invalid_argument("precision must be 0 (fp64), 1 (fp32) or 2 (bf16)");
 }
 
 __global__ void resume_kernel(ChainState* st, int C) {
@@ -493,6 +509,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
 
   DevBuf<double> work;
   DevBuf<int> act_idx, n_act;
+  const void* xlow = nullptr;
   GlmArgs ga;
   GlmPlan gplan;
   if (!fused && glm_ok) {
@@ -503,6 +520,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     ga.X = M.d_X; ga.y = M.d_y; ga.N = M.glmN; ga.D = M.glmD; ga.C = C; ga.rows_per_chunk = gplan.rows_per_chunk;
     ga.s2inv = M.glm_s2inv; ga.Q = Q.p; ga.ldq = Dp[0]; ga.G = G.p; ga.V = V.p; ga.work = work.p;
     ga.idx = act_idx.p; ga.n_active = n_act.p;
+    if (cfg.precision != 0) xlow = M.low_x(cfg.precision, st);
   }
 
   RunResult res;
@@ -530,7 +548,9 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
           if (t) HIPC(hipEventRecord(ev0, st));
           if (glm_ok) {
             launch_compact_active(dneed.p, C, act_idx.p, n_act.p, st);
-            if (glm_grad_mfma_f64(ga, gplan, st)) throw std::runtime_error("GLM kernel: unsupported D");
+            const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
+                                               : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
+            if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
           }
           else launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
           if (t) { HIPC(hipEventRecord(ev1, st)); timed = true; }
@@ -613,6 +633,7 @@ const char* failure_text(int f) {
     case FAIL_STEPSIZE_IMPROPER: return "Posterior is improper. Please check your model.";
     case FAIL_STEPSIZE_ZERO: return "No acceptably small step size could be found. Perhaps the posterior is not continuous?";
     case FAIL_METRIC_OVERFLOW: return "Numerical overflow in metric adaptation.";
+    case FAIL_METRIC_NOT_PD: return "Adapted dense metric is not positive definite.";
     default: return "";
   }
 }
@@ -739,8 +760,8 @@ int nuts_model_unconstrained_name(const nuts_model* m, int dataset, int i, char*
   return guarded([&]() { return copy_name(m->impl.ds.at(dataset).unames.at(i), buf, buflen); });
 }
 
-int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points, int jacobian, double* lp,
-                       double* grad) {
+static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n_points, int jacobian,
+                              int precision, double* lp, double* grad) {
   return guarded([&]() {
     ModelImpl& M = m->impl;
     if (dataset < 0 || dataset >= (int)M.ds.size()) throw std::out_of_range("dataset index");
@@ -760,11 +781,15 @@ int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points
     DevBuf<double> Q(hq.size()), G(hq.size()), V(C);
     dD.upload(Dv); doff.upload(off); ddso.upload(dso); Q.upload(hq);
     ModelDataDev md{M.d_data.p, ddso.p, jacobian ? 1.0 : 0.0};
+    if (precision != 0 && !(M.glm && dataset == 0))
+      throw std::invalid_argument("reduced precision is available for the batched logistic regression only");
     if (M.glm && dataset == 0) {
       const GlmPlan gp = glm_mfma_plan(M.glmN, C, D, num_cus());
       DevBuf<double> work(gp.work_doubles);
       GlmArgs ga{M.d_X, M.d_y, M.glmN, D, C, gp.rows_per_chunk, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
-      if (glm_grad_mfma_f64(ga, gp, 0)) throw std::runtime_error("GLM kernel: unsupported D");
+      const int rcg = precision == 0 ? glm_grad_mfma_f64(ga, gp, 0)
+                                     : glm_grad_mfma_f32(ga, gp, M.low_x(precision, 0), precision, kF32Layout, 0);
+      if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
       HIPC(hipDeviceSynchronize());
     } else {
       launch_model_grad(M.kind, C, dD.p, doff.p, Q.p, G.p, V.p, nullptr, md, 0);
@@ -781,13 +806,23 @@ int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points
   });
 }
 
+int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points, int jacobian, double* lp,
+                       double* grad) {
+  return log_prob_grad_impl(m, dataset, q, n_points, jacobian, 0, lp, grad);
+}
+
+int nuts_log_prob_grad_precision(nuts_model* m, int dataset, const double* q, int n_points, int jacobian,
+                                 int precision, double* lp, double* grad) {
+  return log_prob_grad_impl(m, dataset, q, n_points, jacobian, precision, lp, grad);
+}
+
 void nuts_config_default(nuts_config* c) {
   std::memset(c, 0, sizeof(*c));
   c->num_samples = 1000; c->num_warmup = 1000; c->save_warmup = 0; c->thin = 1;
   c->adapt_engaged = 1; c->gamma = 0.05; c->delta = 0.8; c->kappa = 0.75; c->t0 = 10;
   c->init_buffer = 75; c->term_buffer = 50; c->window = 25;
   c->max_depth = 10; c->metric = NUTS_METRIC_DIAG_E; c->stepsize = 1; c->stepsize_jitter = 0;
-  c->num_chains = 1; c->id = 1; c->seed = 0; c->init_radius = 2; c->fused = -1;
+  c->num_chains = 1; c->id = 1; c->seed = 0; c->init_radius = 2; c->fused = -1; c->precision = 0;
 }
 
 int nuts_config_validate(const nuts_config* cfg) {

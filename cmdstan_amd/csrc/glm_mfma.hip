@@ -27,11 +27,13 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 #define GM_KMAX 128         // D capacity (K of phase 1, N of phase 2)
 #define GM_XS (GM_KMAX + 2) // LDS row stride in doubles (== 2 mod 32: conflict-free phase-1 reads)
 
+template <int KMAX>
 __global__ void __launch_bounds__(256, 2)
 glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, int D, int rows_per_chunk,
              int nchunks, int ngroups, const double* __restrict__ Q, int64_t ldq, int C, const int* __restrict__ idx,
              const int* __restrict__ n_active, double* __restrict__ Wg, double* __restrict__ Wlp) {
   __shared__ double sX[GM_TN * GM_XS];
+  __shared__ double sY[GM_TN];
   const int tid = threadIdx.x;
   const int w = tid >> 6;
   const int l = tid & 63;
@@ -53,19 +55,19 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
   const int DT = (D + 15) >> 4;           // phase-2 output tiles
 
   // phase-1 B operand for all k-steps: Q[c0 + lr][4 ks + lk]
-  double qreg[GM_KMAX / 4];
+  double qreg[KMAX / 4];
   {
     const int pos = c0 + lr;
     const int c = pos < n_act ? (idx ? idx[pos] : pos) : -1;
 #pragma unroll
-    for (int ks = 0; ks < GM_KMAX / 4; ++ks) {
+    for (int ks = 0; ks < KMAX / 4; ++ks) {
       const int k = 4 * ks + lk;
       qreg[ks] = (ks < KS && c >= 0 && k < D) ? Q[(int64_t)c * ldq + k] : 0.0;
     }
   }
-  d4 gacc[GM_KMAX / 16];
+  d4 gacc[KMAX / 16];
 #pragma unroll
-  for (int t = 0; t < GM_KMAX / 16; ++t) gacc[t] = d4{0, 0, 0, 0};
+  for (int t = 0; t < KMAX / 16; ++t) gacc[t] = d4{0, 0, 0, 0};
   double lp = 0.0;
 
   const int n_begin = chunk * rows_per_chunk;
@@ -74,18 +76,27 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
 
   for (int nb = n_begin; nb < n_end; nb += GM_TN) {
     __syncthreads();
-    for (int i = tid; i < GM_TN * ncols; i += 256) {
-      const int r = i / ncols, k = i - r * ncols;
+    // 16 lanes cover 16 consecutive columns of a row: 128-byte segments
+#pragma unroll
+    for (int m = 0; m < GM_TN / 16; ++m) {
+      const int r = (tid >> 4) + 16 * m;
       const int n = nb + r;
-      sX[r * GM_XS + k] = (n < n_end && k < D) ? X[(int64_t)n * D + k] : 0.0;
+      const bool rowok = n < n_end;
+      const double* xr = X + (int64_t)n * D;
+#pragma unroll
+      for (int jj = 0; jj < KMAX / 16; ++jj) {
+        const int k = (tid & 15) + 16 * jj;
+        if (k < ncols) sX[r * GM_XS + k] = (rowok && k < D) ? xr[k] : 0.0;
+      }
     }
+    if (tid < GM_TN) sY[tid] = (nb + tid < n_end) ? y[nb + tid] : 0.0;
     __syncthreads();
 
     d4 eta[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) eta[rb] = d4{0, 0, 0, 0};
 #pragma unroll
-    for (int ks = 0; ks < GM_KMAX / 4; ++ks) {
+    for (int ks = 0; ks < KMAX / 4; ++ks) {
       if (ks < KS) {
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb) {
@@ -99,13 +110,15 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
     for (int rb = 0; rb < 4; ++rb) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int n = nb + 16 * rb + 4 * i + lk;
+        const int rr = 16 * rb + 4 * i + lk;
         const double e = eta[rb][i];
         double r = 0.0;
-        if (n < n_end) {
-          const double yy = y[n];
-          lp += yy * e - log1p_exp(e);
-          r = inv_logit(e) - yy;
+        if (nb + rr < n_end) {
+          // one exp: log1p_exp(e) = max(e,0) + log1p(t), sigmoid(e) from t = e^{-|e|}
+          const double yy = sY[rr];
+          const double t = exp(-fabs(e));
+          lp += yy * e - ((e > 0.0 ? e : 0.0) + log1p(t));
+          r = (e >= 0.0 ? 1.0 / (1.0 + t) : t / (1.0 + t)) - yy;
         }
         eta[rb][i] = r;
       }
@@ -117,7 +130,7 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
         const double a = eta[rb][i];
         const double* xrow = sX + (16 * rb + 4 * i + lk) * GM_XS + lr;
 #pragma unroll
-        for (int t = 0; t < GM_KMAX / 16; ++t) {
+        for (int t = 0; t < KMAX / 16; ++t) {
           if (t < DT) gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xrow[16 * t], gacc[t], 0, 0, 0);
         }
       }
@@ -126,7 +139,7 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
 
   // gacc[t] register i of lane l = G[chain c0 + 4i + lk][d 16t + lr]
 #pragma unroll
-  for (int t = 0; t < GM_KMAX / 16; ++t) {
+  for (int t = 0; t < KMAX / 16; ++t) {
     if (t < DT) {
       const int d = 16 * t + lr;
 #pragma unroll
@@ -230,11 +243,25 @@ int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
   double* Wg = a.work;
   double* Wlp = a.work + (size_t)p.nchunks * a.C * a.D;
   const int nblocks = (p.nchunks / 8) * p.ngroups * 8;
-  hipLaunchKernelGGL(glm_mfma_f64, dim3(nblocks), dim3(256), 0, st, a.X, a.y, a.N, a.D, p.rows_per_chunk, p.nchunks,
-                     p.ngroups, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp);
+#define GM_LAUNCH(K)                                                                                            \
+  hipLaunchKernelGGL(glm_mfma_f64<K>, dim3(nblocks), dim3(256), 0, st, a.X, a.y, a.N, a.D, p.rows_per_chunk,    \
+                     p.nchunks, p.ngroups, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp)
+  // operand registers scale with the D bucket
+  if (a.D <= 16) GM_LAUNCH(16);
+  else if (a.D <= 32) GM_LAUNCH(32);
+  else if (a.D <= 64) GM_LAUNCH(64);
+  else if (a.D <= 112) GM_LAUNCH(112);
+  else GM_LAUNCH(128);
+#undef GM_LAUNCH
+  glm_reduce_launch(a, p, st);
+  return 0;
+}
+
+void glm_reduce_launch(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
+  double* Wg = a.work;
+  double* Wlp = a.work + (size_t)p.nchunks * a.C * a.D;
   hipLaunchKernelGGL(glm_mfma_reduce, dim3(a.C), dim3(128), 0, st, Wg, Wlp, p.nchunks, a.C, a.D, a.s2inv, a.Q, a.ldq,
                      a.idx, a.n_active, a.G, a.V);
-  return 0;
 }
 
 }  // namespace nutsgpu

@@ -62,6 +62,14 @@ struct GlmPlan {
 };
 GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus);
 void launch_compact_active(int* need, int C, int* idx, int* n_active, hipStream_t st);
+void glm_reduce_launch(const GlmArgs& a, const GlmPlan& p, hipStream_t st);
+
+// FP32 matrix-core version; X pre-converted to FP32 (precision 1) or BF16
+// (precision 2) by glm_convert_x.  `layout` selects the measured register
+// layout of the 16x16 f32 result.
+void glm_convert_x(const double* X, void* out, int64_t n, int precision, hipStream_t st);
+int glm_grad_mfma_f32(const GlmArgs& a, const GlmPlan& p, const void* Xlow, int precision, int layout,
+                      hipStream_t st);
 int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st);
 
 }  // namespace nutsgpu

@@ -600,7 +600,10 @@ struct NutsWave {
       s.welford_n = 0;
       s.w_counter++;
       if (!wave_and(ok)) { fail(FAIL_METRIC_OVERFLOW); return false; }
-      if (DENSE && !dense_factor(dM, dL, D, lane)) { fail(FAIL_METRIC_OVERFLOW); return false; }
+      if (DENSE) {
+        wave_sync();  // the factorisation reads elements other lanes just wrote
+        if (!dense_factor(dM, dL, D, lane)) { fail(FAIL_METRIC_NOT_PD); return false; }
+      }
       return true;
     }
     s.w_counter++;

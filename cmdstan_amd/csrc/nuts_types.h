@@ -38,6 +38,7 @@ enum Failure : int {
   FAIL_STEPSIZE_IMPROPER = 2,// "Posterior is improper. Please check your model."
   FAIL_STEPSIZE_ZERO = 3,    // "No acceptably small step size could be found..."
   FAIL_METRIC_OVERFLOW = 4,  // "Numerical overflow in metric adaptation."
+  FAIL_METRIC_NOT_PD = 5,    // adapted dense metric has no Cholesky factor
 };
 
 enum SsReturn : int { SS_RET_TRANS = 0, SS_RET_WINDOW = 1 };

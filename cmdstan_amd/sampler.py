@@ -106,21 +106,26 @@ class Model:
     def dim(self):
         return self.dims[0]
 
-    def log_prob_grad(self, q, jacobian=True, dataset=0):
+    def log_prob_grad(self, q, jacobian=True, dataset=0, precision=0):
         q = _f64(np.atleast_2d(q))
         n, D = q.shape
         if D != self.dims[dataset]:
             raise ValueError("dimension mismatch")
         lp = np.zeros(n)
         g = np.zeros((n, D))
-        if lib().nuts_log_prob_grad(self.h, dataset, _dp(q), n, int(jacobian), _dp(lp), _dp(g)) != 0:
+        if precision in (0, "fp64"):
+            rc = lib().nuts_log_prob_grad(self.h, dataset, _dp(q), n, int(jacobian), _dp(lp), _dp(g))
+        else:
+            p = {"fp32": 1, "bf16": 2}.get(precision, precision)
+            rc = lib().nuts_log_prob_grad_precision(self.h, dataset, _dp(q), n, int(jacobian), p, _dp(lp), _dp(g))
+        if rc != 0:
             raise SamplerError(last_error())
         return lp, g
 
     def sample(self, num_chains=1, num_samples=1000, num_warmup=1000, save_warmup=False, thin=1,
                adapt_engaged=True, delta=0.8, gamma=0.05, kappa=0.75, t0=10.0, init_buffer=75, term_buffer=50,
                window=25, max_depth=10, metric="diag_e", stepsize=1.0, stepsize_jitter=0.0, id=1, seed=0,
-               init=2.0, inv_metric=None, fused=-1):
+               init=2.0, inv_metric=None, fused=-1, precision=0):
         cfg = nuts_config()
         lib().nuts_config_default(C.byref(cfg))
         cfg.num_samples, cfg.num_warmup, cfg.save_warmup, cfg.thin = num_samples, num_warmup, int(save_warmup), thin
@@ -129,6 +134,7 @@ class Model:
         cfg.max_depth, cfg.metric = max_depth, METRICS[metric] if isinstance(metric, str) else metric
         cfg.stepsize, cfg.stepsize_jitter = stepsize, stepsize_jitter
         cfg.num_chains, cfg.id, cfg.seed, cfg.fused = num_chains, id, seed, fused
+        cfg.precision = {"fp64": 0, "fp32": 1, "bf16": 2}.get(precision, precision)
         init_arr = None
         if isinstance(init, (int, float)):
             cfg.init_radius = float(init)

@@ -70,6 +70,12 @@ int nuts_model_unconstrained_name(const nuts_model* m, int dataset, int i, char*
 int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points, int jacobian, double* lp,
                        double* grad);
 
+/* The same at the reduced precisions of the batched GLM path (precision 1:
+ * FP32 contraction, X in FP32; 2: X stored in BF16), for error statements
+ * against FP64.  No counterpart in the reference. */
+int nuts_log_prob_grad_precision(nuts_model* m, int dataset, const double* q, int n_points, int jacobian,
+                                 int precision, double* lp, double* grad);
+
 /* -------------------------------------------------------------- sample */
 /* Argument tree of `sample ... algorithm=hmc engine=nuts` with CmdStan defaults
  * (arg_sample.hpp:18-29, arg_adapt.hpp:19-37, arg_hmc.hpp:19-27,
@@ -96,6 +102,8 @@ typedef struct nuts_config {
   uint64_t seed;          /* random seed */
   double init_radius;     /* 2 */
   int fused;              /* -1 auto, 0 lockstep (batched gradient kernels), 1 fused per-chain */
+  int precision;          /* batched GLM contraction: 0 FP64 (default), 1 FP32 with X in FP32,
+                             2 FP32 with X stored in BF16; sums, H, U-turn and tree weights stay FP64 */
 } nuts_config;
 
 void nuts_config_default(nuts_config* cfg);
@@ -115,7 +123,7 @@ int nuts_output_rows(const nuts_config* cfg);
  *   n_leapfrog:[chains] total leapfrog steps; grad_evals: [chains] all gradient
  *              evaluations (initialisation and step-size search included)
  *   status:    [chains] 0 ok, else a failure code (1 init, 2 improper posterior,
- *              3 zero step size, 4 metric overflow) */
+ *              3 zero step size, 4 metric overflow, 5 dense metric not positive definite) */
 typedef struct nuts_output {
   double* draws;
   double* stepsize;

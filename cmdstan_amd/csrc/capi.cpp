@@ -417,6 +417,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         }
     }
     minv.upload(hm, st);
+    HIPC(hipStreamSynchronize(st));  // hm is released at the end of this scope
   }
   DevBuf<double> dMb, dLb, dWb;
   if (dense) {
@@ -435,6 +436,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         }
     }
     dMb.upload(hm, st);
+    HIPC(hipStreamSynchronize(st));
     dLb.alloc(mtot);
     dWb.alloc(mtot);
     HIPC(hipMemsetAsync(dWb.p, 0, mtot * sizeof(double), st));
@@ -444,6 +446,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     for (int c = 0; c < C; ++c)
       for (int i = 0; i < D[c]; ++i) hi[off[c] + i] = R.init[(size_t)c * D[0] + i];
     uinit.upload(hi, st);
+    HIPC(hipStreamSynchronize(st));
   }
   if (R.inj_u) { du.alloc((size_t)C * std::max<int64_t>(R.n_u, 1)); du.upload(R.inj_u, (size_t)C * R.n_u, st); }
   if (R.inj_z) { dz.alloc((size_t)C * std::max<int64_t>(R.n_z, 1)); dz.upload(R.inj_z, (size_t)C * R.n_z, st); }

@@ -1,9 +1,10 @@
 // Batched logistic-regression gradient with the contraction on FP32 matrix
 // cores (v_mfma_f32_16x16x4f32, FP32 accumulation) and X stored in FP32 or
-// BF16 (converted to FP32 on its way into LDS).  The per-chain log density is
-// summed in FP64 from the FP32 linear predictor, and chunk partials of the
-// gradient are reduced in FP64, so V and ∇V reach the FP64 sampler with an
-// error set by the FP32 products alone.
+// BF16 (converted to FP32 on its way into LDS).  The per-observation terms are
+// formed in FP32 from the FP32 linear predictor and the per-chain log density
+// is summed in FP64; chunk partials of the gradient are reduced in FP64, so V
+// and ∇V reach the FP64 sampler with an error set by FP32 rounding of the
+// products and terms (stated against FP64 in the tests).
 //
 // Same structure as the FP64 kernel: a block owns (row chunk, 64 chains), four
 // waves of 16 chains; phase 1 Eta = X·Qᵀ, pointwise R, phase 2 Gᵀ += Rᵀ·X with
@@ -109,13 +110,15 @@ glm_mfma_f32(const XT* __restrict__ X, const double* __restrict__ y, int N, int 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int rr = 16 * rb + f32_row<LAYOUT>(l, i);
-        const double e = (double)eta[rb][i];
+        const float e = eta[rb][i];
         float r = 0.0f;
         if (nb + rr < n_end) {
-          const double yy = sY[rr];
-          const double t = exp(-fabs(e));
-          lp += yy * e - ((e > 0.0 ? e : 0.0) + log1p(t));
-          r = (float)((e >= 0.0 ? 1.0 / (1.0 + t) : t / (1.0 + t)) - yy);
+          // terms in FP32 from the FP32 linear predictor, summed in FP64
+          const float yy = (float)sY[rr];
+          const float t = __expf(-fabsf(e));
+          const float l1p = t < 1e-4f ? t * (1.0f - 0.5f * t) : __logf(1.0f + t);
+          lp += (double)(yy * e - ((e > 0.0f ? e : 0.0f) + l1p));
+          r = (e >= 0.0f ? 1.0f : t) / (1.0f + t) - yy;
         }
         eta[rb][i] = r;
       }

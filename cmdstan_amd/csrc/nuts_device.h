@@ -40,13 +40,16 @@ struct ExternalGrad {
 // D <= 64 * DENSE_RMAX
 #define DENSE_RMAX 16
 
-// Orders this lane's earlier global/LDS writes before later reads of other
-// lanes of the same wavefront (vectors are written lane-strided and then read
-// whole by every lane in matvecs and model gradients).
+// Makes this wave's earlier global writes visible to later reads by other
+// lanes of the same wave (vectors are written lane-strided and then read whole
+// by every lane in matvecs, the Welford covariance update, the Cholesky
+// columns and the model gradients).  Agent scope: the release waits for the
+// stores, the acquire invalidates the CU's vector L1 so a line cached before
+// the stores is not read back stale.
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 template <int METRIC, class GradFn>
@@ -65,6 +68,11 @@ struct NutsWave {
   double* dM;    // dense: inverse metric, column-major D×D
   double* dL;    // dense: lower Cholesky factor, column-major
   double* dW;    // dense: Welford m2, column-major
+  // V at Q of the pending gradient: computed in this launch (fused: kept in
+  // registers, identical on every lane) or by the gradient kernel of the
+  // previous launch (read from memory at the kernel boundary)
+  double v_new = 0;
+  bool have_v = false;
 
   __device__ NutsWave(const NutsParams& P_, const NutsBuffers& B_, ChainState& s_, const GradFn& g_, int c_, int lane_)
       : P(P_), B(B_), s(s_), grad(g_), c(c_), lane(lane_) {
@@ -242,6 +250,8 @@ struct NutsWave {
     return k;
   }
 
+  __device__ double pending_V() const { return have_v ? v_new : B.V[c]; }
+
   // -------------------------------------------------------- control flow
   __device__ bool request(int next) {
     s.phase = next;
@@ -270,7 +280,7 @@ struct NutsWave {
     return request(PH_INIT_CHECK);
   }
   __device__ bool init_check() {
-    const double V = B.V[c];
+    const double V = pending_V();
     int ok = 1;
     for (int i = lane; i < D; i += NUTS_WAVE) ok &= isfinite(g[i]) ? 1 : 0;
     ok = wave_and(ok) && isfinite(V);
@@ -310,7 +320,7 @@ struct NutsWave {
     return ss_probe_begin();
   }
   __device__ bool ss_probe() {
-    s.Vz = B.V[c];
+    s.Vz = pending_V();
     leap_end(s.nom_eps);
     double h = kinetic(vecp(PV_P), vecp(PV_PS)) + s.Vz;
     if (isnan(h)) h = __builtin_inf();
@@ -404,7 +414,7 @@ struct NutsWave {
   // build_tree base case plus the merges that close at this leaf
   __device__ bool leaf() {
     const double e = s.dir * s.eps;
-    s.Vz = B.V[c];
+    s.Vz = pending_V();
     leap_end(e);
     double* p = vecp(PV_P);
     double* ps = vecp(PV_PS);
@@ -602,7 +612,18 @@ struct NutsWave {
       if (!wave_and(ok)) { fail(FAIL_METRIC_OVERFLOW); return false; }
       if (DENSE) {
         wave_sync();  // the factorisation reads elements other lanes just wrote
-        if (!dense_factor(dM, dL, D, lane)) { fail(FAIL_METRIC_NOT_PD); return false; }
+        if (!dense_factor(dM, dL, D, lane)) {
+#ifdef NUTS_DEBUG_DENSE
+          if (lane == 0 && D <= 4) {
+            printf("dense not PD: chain %d iter %d n %d wcounter %u\n", c, s.iter, (int)n, s.w_counter);
+            for (int i = 0; i < D; ++i)
+              printf("  M row %d: %g %g %g %g | m %g\n", i, dM[i], D > 1 ? dM[D + i] : 0.0, D > 2 ? dM[2 * D + i] : 0.0,
+                     D > 3 ? dM[3 * D + i] : 0.0, wm[i]);
+          }
+#endif
+          fail(FAIL_METRIC_NOT_PD);
+          return false;
+        }
       }
       return true;
     }
@@ -681,10 +702,12 @@ struct NutsWave {
       const bool req = dispatch();
       if (!req) continue;
       if (!GradFn::fused) { if (lane == 0) B.need[c] = 1; return; }
-      wave_sync();
+      wave_sync();  // the model reads q written lane-strided
       const double V = grad(c, q, g, D, lane);
-      wave_sync();
-      if (lane == 0) B.V[c] = V;
+      v_new = V;
+      have_v = true;
+      if (lane == 0) B.V[c] = V;  // for the next launch only
+      wave_sync();  // models may write g elements that other lanes read
       if (--budget == 0) return;
     }
   }

@@ -300,3 +300,16 @@ def test_reduced_precision_posterior_matches_fp64(precision):
         assert sb["rhat"] < 1.01
         assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"])
         assert abs(sa["sd"] - sb["sd"]) < 0.05 * sa["sd"]
+
+
+def test_dense_deterministic_and_valid_over_seeds():
+    """dense_e adaptation over several seeds: every chain finishes, and a
+    repeated run is bit-identical (no read of a stale vector across lanes)."""
+    Sig = np.array([[1.0, 0.8, 0.0], [0.8, 1.0, 0.3], [0.0, 0.3, 2.0]])
+    gm = ca.Model("gauss", dict(D=3, P=np.linalg.inv(Sig)))
+    for seed in (1, 2, 3):
+        a = gm.sample(num_chains=32, seed=seed, metric="dense_e", num_warmup=300, num_samples=50)
+        b = gm.sample(num_chains=32, seed=seed, metric="dense_e", num_warmup=300, num_samples=50)
+        assert np.all(a.status == 0)
+        for x, y in zip(a.draws, b.draws):
+            assert np.array_equal(x, y)

@@ -20,7 +20,7 @@ class nuts_config(C.Structure):
         ("t0", C.c_double), ("init_buffer", C.c_int), ("term_buffer", C.c_int), ("window", C.c_int),
         ("max_depth", C.c_int), ("metric", C.c_int), ("stepsize", C.c_double), ("stepsize_jitter", C.c_double),
         ("num_chains", C.c_int), ("id", C.c_int), ("seed", C.c_uint64), ("init_radius", C.c_double),
-        ("fused", C.c_int), ("precision", C.c_int),
+        ("fused", C.c_int), ("precision", C.c_int), ("max_saved_params", C.c_int),
     ]
 
 

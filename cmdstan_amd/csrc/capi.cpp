@@ -360,6 +360,8 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   std::vector<int64_t> off(C), moff(C), doff(C), dsoff(C), ooff(C);
   std::vector<uint32_t> uid(C);
   int64_t tot = 0, dtot = 0, otot = 0;
+  const int msp = cfg.max_saved_params;
+  auto nsave_of = [msp](int d) { return msp < 0 ? d : std::min(d, msp); };
   for (int c = 0; c < C; ++c) {
     const int m = R.first_dataset + c / cfg.num_chains;
     D[c] = M.ds[m].D;
@@ -369,7 +371,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     moff[c] = 0;
     tot += Dp[c];
     doff[c] = dtot;
-    dtot += (int64_t)rows * (7 + D[c]);
+    dtot += (int64_t)rows * (7 + nsave_of(D[c]));
     ooff[c] = otot;
     otot += (int64_t)rows * (7 + nout[c]);
     dsoff[c] = M.ds_off[m];
@@ -483,6 +485,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   P.seed = cfg.seed;
   P.nslots = nslots; P.maxd = maxd; P.NV = NV;
   P.fused_budget = 256;
+  P.save_params = msp;
 
   NutsBuffers B;
   std::memset(&B, 0, sizeof(B));
@@ -610,7 +613,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     HIPC(hipStreamSynchronize(st));
     for (int c = 0; c < C; ++c) for (int i = 0; i < D[c]; ++i) res.minv.push_back(hm[off[c] + i]);
   }
-  if (R.constrain) {
+  if (R.constrain && msp < 0) {
     DevBuf<double> out(std::max<int64_t>(otot, 1));
     DevBuf<int> dn(C);
     dn.upload(nout, st);
@@ -623,7 +626,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     res.draws.resize(dtot);
     HIPC(hipMemcpyAsync(res.draws.data(), draws.p, dtot * sizeof(double), hipMemcpyDeviceToHost, st));
     res.draw_off = doff;
-    for (int c = 0; c < C; ++c) res.cols.push_back(7 + D[c]);
+    for (int c = 0; c < C; ++c) res.cols.push_back(7 + nsave_of(D[c]));
   }
   HIPC(hipStreamSynchronize(st));
   HIPC(hipGetLastError());
@@ -826,6 +829,7 @@ void nuts_config_default(nuts_config* c) {
   c->init_buffer = 75; c->term_buffer = 50; c->window = 25;
   c->max_depth = 10; c->metric = NUTS_METRIC_DIAG_E; c->stepsize = 1; c->stepsize_jitter = 0;
   c->num_chains = 1; c->id = 1; c->seed = 0; c->init_radius = 2; c->fused = -1; c->precision = 0;
+  c->max_saved_params = -1;
 }
 
 int nuts_config_validate(const nuts_config* cfg) {

@@ -36,8 +36,8 @@ struct ExternalGrad {
   __device__ double operator()(int, const double*, double*, int, int) const { return 0.0; }
 };
 
-// dense_e keeps one momentum in registers during the triangular solve:
-// D <= 64 * DENSE_RMAX
+// dense_e keeps one momentum in an LDS strip of the wave during the
+// triangular solve: D <= 64 * DENSE_RMAX
 #define DENSE_RMAX 16
 
 // Makes this wave's earlier global writes visible to later reads by other
@@ -46,6 +46,12 @@ struct ExternalGrad {
 // columns and the model gradients).  Agent scope: the release waits for the
 // stores, the acquire invalidates the CU's vector L1 so a line cached before
 // the stores is not read back stale.
+// Orders LDS writes of this wave before later LDS reads by its other lanes.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __builtin_amdgcn_wave_barrier();
@@ -165,38 +171,25 @@ struct NutsWave {
       advance_normals();
       return;
     }
-    // back substitution Lᵀ p = z (U = Lᵀ upper) with p held in registers:
-    // lane l owns elements l + 64 r; column i of L below the diagonal is
-    // contiguous in the column-major factor
-    double pr[DENSE_RMAX];
-#pragma unroll
-    for (int r = 0; r < DENSE_RMAX; ++r) {
-      const int i = lane + NUTS_WAVE * r;
-      pr[r] = i < D ? normal_draw(i) : 0.0;
-    }
+    // back substitution Lᵀ p = z (U = Lᵀ upper) in an LDS strip owned by this
+    // wave; column i of L below the diagonal is contiguous in the
+    // column-major factor
+    __shared__ double sp[4][NUTS_WAVE * DENSE_RMAX];
+    double* pw = sp[threadIdx.x >> 6];
+    for (int i = lane; i < D; i += NUTS_WAVE) pw[i] = normal_draw(i);
     advance_normals();
+    lds_sync();
     for (int i = D - 1; i >= 0; --i) {
       const double* Li = dL + (int64_t)i * D;
-      const int ri = i >> 6, li = i & 63;
-      double a = 0, zi = 0;
-#pragma unroll
-      for (int r = 0; r < DENSE_RMAX; ++r) {
-        const int j = lane + NUTS_WAVE * r;
-        if (j > i && j < D) a += Li[j] * pr[r];
-        if (r == ri) zi = pr[r];
-      }
+      double a = 0;
+      for (int j = i + 1 + lane; j < D; j += NUTS_WAVE) a += Li[j] * pw[j];
       a = wave_sum(a);
-      zi = __shfl(zi, li, NUTS_WAVE);
-      const double pi = (zi - a) / Li[i];
-#pragma unroll
-      for (int r = 0; r < DENSE_RMAX; ++r)
-        if (r == ri && lane == li) pr[r] = pi;
+      const double pi = (pw[i] - a) / Li[i];
+      lds_sync();  // every lane has read z_i before it is overwritten
+      if (lane == 0) pw[i] = pi;
+      lds_sync();
     }
-#pragma unroll
-    for (int r = 0; r < DENSE_RMAX; ++r) {
-      const int i = lane + NUTS_WAVE * r;
-      if (i < D) p[i] = pr[r];
-    }
+    for (int i = lane; i < D; i += NUTS_WAVE) p[i] = pw[i];
     wave_sync();
     matvec(p, ps);
   }
@@ -613,14 +606,6 @@ struct NutsWave {
       if (DENSE) {
         wave_sync();  // the factorisation reads elements other lanes just wrote
         if (!dense_factor(dM, dL, D, lane)) {
-#ifdef NUTS_DEBUG_DENSE
-          if (lane == 0 && D <= 4) {
-            printf("dense not PD: chain %d iter %d n %d wcounter %u\n", c, s.iter, (int)n, s.w_counter);
-            for (int i = 0; i < D; ++i)
-              printf("  M row %d: %g %g %g %g | m %g\n", i, dM[i], D > 1 ? dM[D + i] : 0.0, D > 2 ? dM[2 * D + i] : 0.0,
-                     D > 3 ? dM[3 * D + i] : 0.0, wm[i]);
-          }
-#endif
           fail(FAIL_METRIC_NOT_PD);
           return false;
         }
@@ -639,7 +624,8 @@ struct NutsWave {
     const bool warm = s.iter < P.num_warmup;
     const bool save = warm ? (P.save_warmup && (s.iter % P.thin) == 0) : (((s.iter - P.num_warmup) % P.thin) == 0);
     if (save) {
-      double* row = B.draws + B.draw_off[c] + (int64_t)s.draw_idx * (7 + D);
+      const int nsave = P.save_params < 0 ? D : (P.save_params < D ? P.save_params : D);
+      double* row = B.draws + B.draw_off[c] + (int64_t)s.draw_idx * (7 + nsave);
       if (lane == 0) {
         row[0] = -s.slot_V[k];
         row[1] = accept;
@@ -649,7 +635,8 @@ struct NutsWave {
         row[5] = s.divergent;
         row[6] = s.slot_H[k];
       }
-      copy(row + 7, Sq(k));
+      const double* sq = Sq(k);
+      for (int i = lane; i < nsave; i += NUTS_WAVE) row[7 + i] = sq[i];
       s.draw_idx++;
     }
     if (s.adapt_flag) {

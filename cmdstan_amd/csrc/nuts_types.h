@@ -78,6 +78,7 @@ struct NutsParams {
   uint32_t w_counter0, w_size0, w_next0, metric_adapt;
   uint64_t seed;
   int nslots, maxd, NV, fused_budget;
+  int save_params, pad_p;  // -1: all D coordinates in each draw row; else the first save_params
 };
 
 // Device pointers (kernel argument by value).

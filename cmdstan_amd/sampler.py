@@ -125,7 +125,7 @@ class Model:
     def sample(self, num_chains=1, num_samples=1000, num_warmup=1000, save_warmup=False, thin=1,
                adapt_engaged=True, delta=0.8, gamma=0.05, kappa=0.75, t0=10.0, init_buffer=75, term_buffer=50,
                window=25, max_depth=10, metric="diag_e", stepsize=1.0, stepsize_jitter=0.0, id=1, seed=0,
-               init=2.0, inv_metric=None, fused=-1, precision=0):
+               init=2.0, inv_metric=None, fused=-1, precision=0, max_saved_params=-1):
         cfg = nuts_config()
         lib().nuts_config_default(C.byref(cfg))
         cfg.num_samples, cfg.num_warmup, cfg.save_warmup, cfg.thin = num_samples, num_warmup, int(save_warmup), thin
@@ -135,6 +135,7 @@ class Model:
         cfg.stepsize, cfg.stepsize_jitter = stepsize, stepsize_jitter
         cfg.num_chains, cfg.id, cfg.seed, cfg.fused = num_chains, id, seed, fused
         cfg.precision = {"fp64": 0, "fp32": 1, "bf16": 2}.get(precision, precision)
+        cfg.max_saved_params = max_saved_params
         init_arr = None
         if isinstance(init, (int, float)):
             cfg.init_radius = float(init)
@@ -150,7 +151,12 @@ class Model:
             raise SamplerError(last_error())
         rows = lib().nuts_output_rows(C.byref(cfg))
         ds_of_chain = [m for m in range(self.num_datasets) for _ in range(num_chains)]
-        cols = [7 + self.num_outputs[m] for m in ds_of_chain]
+        if max_saved_params < 0:
+            cols = [7 + self.num_outputs[m] for m in ds_of_chain]
+            names = [SAMPLER_COLUMNS + p for p in self.param_names]
+        else:
+            cols = [7 + min(self.dims[m], max_saved_params) for m in ds_of_chain]
+            names = [SAMPLER_COLUMNS + u[:max_saved_params] for u in self.unconstrained_names]
         total = sum(rows * c for c in cols)
         draws = np.zeros(total)
         step = np.zeros(C_tot)
@@ -182,7 +188,7 @@ class Model:
         fit = Fit(draws=per_chain, stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
                   status=status, warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds,
                   leaf_steps=out.leaf_steps, grad_seconds_per_launch=out.grad_seconds_per_launch,
-                  grad_launches_timed=out.grad_launches_timed, column_names=[SAMPLER_COLUMNS + p for p in self.param_names],
+                  grad_launches_timed=out.grad_launches_timed, column_names=names,
                   dataset_of_chain=ds_of_chain, config=cfg, error=err)
         if rc != 0:
             raise SamplerError(err)

@@ -104,6 +104,9 @@ typedef struct nuts_config {
   int fused;              /* -1 auto, 0 lockstep (batched gradient kernels), 1 fused per-chain */
   int precision;          /* batched GLM contraction: 0 FP64 (default), 1 FP32 with X in FP32,
                              2 FP32 with X stored in BF16; sums, H, U-turn and tree weights stay FP64 */
+  int max_saved_params;   /* -1 (default): draws hold the constrained parameters; k >= 0: draws hold
+                             the first k unconstrained coordinates only (bounded output for very
+                             large batches) */
 } nuts_config;
 
 void nuts_config_default(nuts_config* cfg);

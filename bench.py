@@ -30,6 +30,33 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the same
+    counters on a copy of known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles",
+                        f"{kernel}_{args.precision}_logistic_N{args.N}_D{args.D}_C{args.chains}_counters.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(nbytes):
+    """A plain device copy of known size: the calibration dispatch for the
+    FETCH_SIZE / WRITE_SIZE counters (read nbytes, write nbytes)."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+
+
 def make_data(N, D, seed):
     rng = np.random.default_rng(seed)
     X = rng.standard_normal((N, D))
@@ -172,12 +199,16 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"],
                     help="c2 (default): the headline workload; c1, c3: secondary measurements")
     ap.add_argument("--models", type=int, default=512, help="c3: number of models")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="first run a device copy of BYTES (counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the chain-sharded multi-GPU design is documented "
                                    "but not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     if args.config == "c1":
         print(json.dumps(bench_c1(args)))
         return
@@ -207,19 +238,19 @@ def main():
     nlf_samp = float(arr[:, :, 4].sum())
     grad_per_s = nlf_samp / samp_s
     value = min_ess / samp_s
-    # roofline of the gradient kernel: 4 N D flops per chain per launch
-    flops_per_launch = 4.0 * args.N * D * C
+    # roofline of the gradient's contraction kernel: 4 N D flops per active chain
+    # per launch (X.B and X^T.R), averaged over every lockstep launch of the run
     roof = None
-    if getattr(fit, "grad_seconds_per_launch", 0):
+    if fit.grad_seconds_per_launch > 0 and fit.grad_chains_per_launch > 0:
         t = fit.grad_seconds_per_launch
+        flops_per_launch = 4.0 * args.N * D * fit.grad_chains_per_launch
         ach = flops_per_launch / t / 1e12
         peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
-        roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak, traffic=None,
-                    kernel="glm_mfma_f64" if args.precision == "fp64" else f"glm_mfma_f32 (X {args.precision})",
-                    launch_ms=t * 1e3)
-    else:
-        # lockstep: every leaf launches one gradient over all chains
-        t = samp_s / max(1, fit.leaf_steps_sampling) if getattr(fit, "leaf_steps_sampling", 0) else None
+        kname = "glm_mfma_f64" if args.precision == "fp64" else "glm_mfma_f32"
+        roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                    traffic=measured_traffic(kname, args), kernel=kname, launch_ms=t * 1e3,
+                    launches=fit.grad_launches_timed, mean_active_chains=fit.grad_chains_per_launch,
+                    flops_per_launch=flops_per_launch)
     cpu = None
     if not args.no_cpu_baseline:
         ess_per_grad = min_ess / nlf_samp

@@ -29,6 +29,7 @@ class nuts_output(C.Structure):
         ("draws", DP), ("stepsize", DP), ("inv_metric", DP), ("n_leapfrog", I64P), ("grad_evals", I64P),
         ("status", C.POINTER(C.c_int)), ("warmup_seconds", C.c_double), ("sampling_seconds", C.c_double),
         ("leaf_steps", C.c_int64), ("grad_seconds_per_launch", C.c_double), ("grad_launches_timed", C.c_int64),
+        ("grad_chains_per_launch", C.c_double),
     ]
 
 

@@ -312,6 +312,7 @@ struct RunResult {
   int64_t leaf_steps = 0;
   double grad_s = 0;
   int64_t grad_timed = 0;
+  long long grad_chains = 0;  // chains evaluated, summed over GLM gradient launches
   int rows = 0;
 };
 
@@ -532,43 +533,61 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   RunResult res;
   res.rows = rows;
   int stopped = 0;
-  hipEvent_t ev0, ev1;
-  HIPC(hipEventCreate(&ev0));
-  HIPC(hipEventCreate(&ev1));
-  struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{ev0, ev1};
+  // one event pair per launch of a check interval brackets the gradient's
+  // contraction kernel (the whole gradient kernel off the GLM path)
+  constexpr int kCheck = 16;
+  struct EvPairs {
+    hipEvent_t a[kCheck] = {}, b[kCheck] = {};
+    ~EvPairs() {
+      for (int k = 0; k < kCheck; ++k) {
+        if (a[k]) (void)hipEventDestroy(a[k]);
+        if (b[k]) (void)hipEventDestroy(b[k]);
+      }
+    }
+  } ev;
+  DevBuf<long long> tally;
+  if (!fused) {
+    for (int k = 0; k < kCheck; ++k) {
+      HIPC(hipEventCreate(&ev.a[k]));
+      HIPC(hipEventCreate(&ev.b[k]));
+    }
+    tally.alloc(1);
+    HIPC(hipMemsetAsync(tally.p, 0, sizeof(long long), st));
+  }
   const char* lim = std::getenv("CMDSTAN_AMD_MAX_SECONDS");
   const double max_seconds = lim ? std::atof(lim) : 3600.0;
   const auto t_begin = std::chrono::steady_clock::now();
-  auto run_phase = [&](bool time_grad) {
-    const int check_every = fused ? 1 : 16;
+  auto run_phase = [&]() {
+    const int check_every = fused ? 1 : kCheck;
     for (;;) {
       if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() > max_seconds)
         throw std::runtime_error("sampler did not finish within CMDSTAN_AMD_MAX_SECONDS");
-      bool timed = false;
       for (int k = 0; k < check_every; ++k) {
         if (fused) {
           launch_advance_fused(M.kind, cfg.metric, P, B, md, st);
         } else {
           launch_advance_external(cfg.metric, P, B, st);
-          const bool t = time_grad && k == check_every / 2;
-          if (t) HIPC(hipEventRecord(ev0, st));
           if (glm_ok) {
-            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, st);
+            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+            ga.ev_begin = ev.a[k];
+            ga.ev_end = ev.b[k];
             const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
                                                : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
             if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+          } else {
+            HIPC(hipEventRecord(ev.a[k], st));
+            launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
+            HIPC(hipEventRecord(ev.b[k], st));
           }
-          else launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
-          if (t) { HIPC(hipEventRecord(ev1, st)); timed = true; }
           res.leaf_steps++;
         }
       }
       HIPC(hipGetLastError());
       HIPC(hipMemcpyAsync(&stopped, dstop.p, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPC(hipStreamSynchronize(st));
-      if (timed) {
+      for (int k = 0; k < (fused ? 0 : check_every); ++k) {
         float ms = 0;
-        HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        HIPC(hipEventElapsedTime(&ms, ev.a[k], ev.b[k]));
         res.grad_s += ms * 1e-3;
         res.grad_timed++;
       }
@@ -578,7 +597,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
 
   HIPC(hipStreamSynchronize(st));
   auto t0 = std::chrono::steady_clock::now();
-  run_phase(false);
+  run_phase();
   auto t1 = std::chrono::steady_clock::now();
   // resume paused chains for the sampling phase
   {
@@ -593,10 +612,14 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   P.iter_target = cfg.num_warmup + cfg.num_samples;
   HIPC(hipStreamSynchronize(st));
   auto t2 = std::chrono::steady_clock::now();
-  run_phase(true);
+  run_phase();
   auto t3 = std::chrono::steady_clock::now();
   res.warm_s = std::chrono::duration<double>(t1 - t0).count();
   res.samp_s = std::chrono::duration<double>(t3 - t2).count();
+  if (tally.p) {
+    HIPC(hipMemcpyAsync(&res.grad_chains, tally.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+  }
 
   res.st.resize(C);
   HIPC(hipMemcpyAsync(res.st.data(), dst.p, C * sizeof(ChainState), hipMemcpyDeviceToHost, st));
@@ -869,6 +892,7 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
       out->leaf_steps = r.leaf_steps;
       out->grad_seconds_per_launch = r.grad_timed ? r.grad_s / r.grad_timed : 0.0;
       out->grad_launches_timed = r.grad_timed;
+      out->grad_chains_per_launch = r.leaf_steps ? (double)r.grad_chains / r.leaf_steps : 0.0;
     }
     for (int c = 0; c < C; ++c)
       if (r.st[c].phase != PH_DONE) {

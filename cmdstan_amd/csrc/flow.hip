@@ -1,0 +1,698 @@
+// Affine-coupling flow kernels: an FP32 matrix-core GEMM (v_mfma_f32_16x16x4f32)
+// with split-K and fused bias / tanh / tanh' epilogues, the coupling
+// element-wise steps, and Adam.  Host class FlowNet in flow.h.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+#include "device_common.h"
+#include "flow.h"
+#include "launch.h"
+
+namespace nutsgpu {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+#define FLOW_S 3.0f  // s = S tanh(s_raw / S)
+
+#define FG_BM 64
+#define FG_BN 64
+#define FG_BK 16
+enum { FG_EPI_NONE = 0, FG_EPI_BIAS = 1, FG_EPI_BIAS_TANH = 2, FG_EPI_DTANH = 3 };
+
+struct GemmDesc {
+  const float* A; int64_t sam, sak;  // A(m, k) = A[m*sam + k*sak]
+  const float* B; int64_t sbk, sbn;  // B(k, n) = B[k*sbk + n*sbn]
+  float* C; int64_t ldc;             // C(m, n) = C[m*ldc + n]
+  int M, N, K;
+  const int* Mdev;                   // optional device row count (<= M)
+  int epi;
+  const float* bias;                 // [N]
+  const float* aux; int64_t ldaux;   // FG_EPI_DTANH: times (1 - aux(m, n)^2)
+  float* part; int splits, kper;     // split-K partials [splits][M][N], kper rows of K per split
+};
+
+__device__ __forceinline__ float fg_epi(const GemmDesc& g, float v, int m, int n) {
+  if (g.epi == FG_EPI_BIAS) return v + g.bias[n];
+  if (g.epi == FG_EPI_BIAS_TANH) return tanhf(v + g.bias[n]);
+  if (g.epi == FG_EPI_DTANH) {
+    const float h = g.aux[(int64_t)m * g.ldaux + n];
+    return v * (1.0f - h * h);
+  }
+  return v;
+}
+
+// 64x64 tile per block of four waves (2x2), each wave 32x32 = 2x2 MFMA tiles.
+// f32 16x16x4 operands: lane l gives A[l%16][k=l/16], B[k=l/16][l%16]; result
+// register i of lane l is D[4(l/16)+i][l%16].
+__global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
+  __shared__ float As[FG_BK][FG_BM + 4];
+  __shared__ float Bs[FG_BK][FG_BN + 4];
+  const int M = g.Mdev ? min(g.M, *g.Mdev) : g.M;
+  const int m0 = blockIdx.y * FG_BM, n0 = blockIdx.x * FG_BN;
+  if (m0 >= M) return;
+  const int split = blockIdx.z;
+  const int kb = split * g.kper;
+  const int ke = min(g.K, kb + g.kper);
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0, 0, 0, 0};
+  const bool a_k = g.sak == 1;
+  const bool b_n = g.sbn == 1;
+  for (int k0 = kb; k0 < ke; k0 += FG_BK) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int mm = a_k ? (t >> 2) : ((t & 15) * 4 + e);
+      const int kk = a_k ? ((t & 3) * 4 + e) : (t >> 4);
+      const int m = m0 + mm, k = k0 + kk;
+      As[kk][mm] = (m < M && k < ke) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int nn = b_n ? ((t & 15) * 4 + e) : (t >> 2);
+      const int kk = b_n ? (t >> 4) : ((t & 3) * 4 + e);
+      const int n = n0 + nn, k = k0 + kk;
+      Bs[kk][nn] = (n < g.N && k < ke) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < FG_BK / 4; ++ks) {
+      const int kr = ks * 4 + (l >> 4);
+      const float a0 = As[kr][wm + (l & 15)], a1 = As[kr][wm + 16 + (l & 15)];
+      const float b0 = Bs[kr][wn + (l & 15)], b1 = Bs[kr][wn + 16 + (l & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * i + 4 * (l >> 4) + r;
+        const int n = n0 + wn + 16 * j + (l & 15);
+        if (m < M && n < g.N) {
+          const float v = acc[i][j][r];
+          if (g.splits > 1) g.part[((int64_t)split * g.M + m) * g.N + n] = v;
+          else g.C[(int64_t)m * g.ldc + n] = fg_epi(g, v, m, n);
+        }
+      }
+}
+
+__global__ void flow_gemm_reduce(GemmDesc g) {
+  const int M = g.Mdev ? min(g.M, *g.Mdev) : g.M;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)M * g.N) return;
+  const int m = (int)(i / g.N), n = (int)(i % g.N);
+  float s = 0.0f;
+  for (int sp = 0; sp < g.splits; ++sp) s += g.part[((int64_t)sp * g.M + m) * g.N + n];
+  g.C[(int64_t)m * g.ldc + n] = fg_epi(g, s, m, n);
+}
+
+// ------------------------------------------------------------ element-wise
+__device__ __forceinline__ int rows_of(const int* n_dev, int n) { return n_dev ? min(n, *n_dev) : n; }
+
+// U0[r][j] = Q[idx[r]][j]
+__global__ void flow_gather_z(const double* __restrict__ Q, int64_t ldq, const int* __restrict__ idx,
+                              const int* __restrict__ n_dev, int n, int D, float* __restrict__ U0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= rows_of(n_dev, n)) return;
+  const int64_t c = idx ? idx[r] : r;
+  U0[i] = (float)Q[c * ldq + j];
+}
+
+// UA[r][j] = U[r][ia[j]]
+__global__ void flow_gather_cols(const float* __restrict__ U, int D, const int* __restrict__ ia, int Da,
+                                 const int* __restrict__ n_dev, int n, float* __restrict__ UA) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / Da), j = (int)(i % Da);
+  if (r >= rows_of(n_dev, n)) return;
+  UA[i] = U[(int64_t)r * D + ia[j]];
+}
+
+// forward coupling: Un = U with b-part u_b e^s + t; S[r][j] = s
+__global__ void flow_couple_fwd(const float* __restrict__ U, const float* __restrict__ O, const int* __restrict__ ia,
+                                const int* __restrict__ ib, int Da, int Db, int D, const int* __restrict__ n_dev, int n,
+                                float* __restrict__ S, float* __restrict__ Un) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= rows_of(n_dev, n)) return;
+  const int64_t row = (int64_t)r * D;
+  if (j < Da) {
+    Un[row + ia[j]] = U[row + ia[j]];
+  } else {
+    const int jb = j - Da, d = ib[jb];
+    const float* o = O + (int64_t)r * 2 * Db;
+    const float s = FLOW_S * tanhf(o[jb] / FLOW_S);
+    S[(int64_t)r * Db + jb] = s;
+    Un[row + d] = U[row + d] * expf(s) + o[Db + jb];
+  }
+}
+
+// inverse coupling (training): Vn = V with b-part (v_b - t) e^{-s}
+__global__ void flow_couple_inv(const float* __restrict__ V, const float* __restrict__ O, const int* __restrict__ ia,
+                                const int* __restrict__ ib, int Da, int Db, int D, int n, float* __restrict__ S,
+                                float* __restrict__ Vn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= n) return;
+  const int64_t row = (int64_t)r * D;
+  if (j < Da) {
+    Vn[row + ia[j]] = V[row + ia[j]];
+  } else {
+    const int jb = j - Da, d = ib[jb];
+    const float* o = O + (int64_t)r * 2 * Db;
+    const float s = FLOW_S * tanhf(o[jb] / FLOW_S);
+    S[(int64_t)r * Db + jb] = s;
+    Vn[row + d] = (V[row + d] - o[Db + jb]) * expf(-s);
+  }
+}
+
+// sum over layers of sum_j s, plus sum log scale (FP64), one wave per row
+__global__ void flow_logdet(const float* __restrict__ S, int64_t layer_stride, const int* __restrict__ Db, int L,
+                            const int* __restrict__ n_dev, int n, double log_scale_sum, double* __restrict__ logdet,
+                            int sign) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (r >= rows_of(n_dev, n)) return;
+  double acc = 0;
+  for (int l = 0; l < L; ++l) {
+    const int db = Db[l];
+    const float* s = S + l * layer_stride + (int64_t)r * db;
+    for (int j = lane; j < db; j += NUTS_WAVE) acc += (double)s[j];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) logdet[r] = sign > 0 ? acc + log_scale_sum : -acc - log_scale_sum;
+}
+
+// X = loc + scale * U (FP64 out)
+__global__ void flow_to_x(const float* __restrict__ U, const double* __restrict__ loc, const double* __restrict__ scale,
+                          int D, const int* __restrict__ n_dev, int n, double* __restrict__ X) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= rows_of(n_dev, n)) return;
+  X[i] = loc[j] + scale[j] * (double)U[i];
+}
+
+// V0 = (X - loc) / scale (FP32), rows r < n; rows of X picked by `pick` (or r)
+__global__ void flow_standardize(const double* __restrict__ X, const int64_t* __restrict__ pick,
+                                 const double* __restrict__ loc, const double* __restrict__ scale, int D, int n,
+                                 float* __restrict__ V0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= n) return;
+  const int64_t src = pick ? pick[r] : r;
+  V0[i] = (float)((X[src * D + j] - loc[j]) / scale[j]);
+}
+
+// GC = GX * scale (cotangent of the last layer's output)
+__global__ void flow_bwd_init(const double* __restrict__ GX, const double* __restrict__ scale, int D,
+                              const int* __restrict__ n_dev, int n, float* __restrict__ GC) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= rows_of(n_dev, n)) return;
+  GC[i] = (float)(GX[i] * scale[j]);
+}
+
+// sampling VJP through one coupling: GO = [dU/ds_raw, dU/dt]; GC_b <- gv_b e^s
+// (U = V(T(z)) - log|det|, so dU/ds = gv_b u_b e^s - 1)
+__global__ void flow_couple_bwd_pull(float* __restrict__ GC, const float* __restrict__ U, const float* __restrict__ S,
+                                     const int* __restrict__ ib, int Db, int D, const int* __restrict__ n_dev, int n,
+                                     float* __restrict__ GO) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / Db), j = (int)(i % Db);
+  if (r >= rows_of(n_dev, n)) return;
+  const int64_t e = (int64_t)r * D + ib[j];
+  const float s = S[i];
+  const float es = expf(s);
+  const float gv = GC[e];
+  const float gs = gv * U[e] * es - 1.0f;
+  const float q = s / FLOW_S;
+  GO[(int64_t)r * 2 * Db + j] = gs * (1.0f - q * q);
+  GO[(int64_t)r * 2 * Db + Db + j] = gv;
+  GC[e] = gv * es;
+}
+
+// training backward through one inverse coupling (loss = mean 0.5|z|^2 + sum s):
+// u_b = (v_b - t) e^{-s}: dL/ds = -gu_b u_b + 1/B, dL/dt = -gu_b e^{-s}, gv_b = gu_b e^{-s}
+__global__ void flow_couple_bwd_train(float* __restrict__ GC, const float* __restrict__ Uz, const float* __restrict__ S,
+                                      const int* __restrict__ ib, int Db, int D, int n, float inv_b,
+                                      float* __restrict__ GO) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / Db), j = (int)(i % Db);
+  if (r >= n) return;
+  const int64_t e = (int64_t)r * D + ib[j];
+  const float s = S[i];
+  const float ems = expf(-s);
+  const float gu = GC[e];
+  const float q = s / FLOW_S;
+  GO[(int64_t)r * 2 * Db + j] = (inv_b - gu * Uz[e]) * (1.0f - q * q);
+  GO[(int64_t)r * 2 * Db + Db + j] = -gu * ems;
+  GC[e] = gu * ems;
+}
+
+// GC[r][ia[j]] += GUA[r][j]
+__global__ void flow_scatter_add(float* __restrict__ GC, const float* __restrict__ GUA, const int* __restrict__ ia,
+                                 int Da, int D, const int* __restrict__ n_dev, int n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / Da), j = (int)(i % Da);
+  if (r >= rows_of(n_dev, n)) return;
+  GC[(int64_t)r * D + ia[j]] += GUA[i];
+}
+
+// G[idx[r]] = GC[r]; V[idx[r]] = VX[r] - logdet[r]
+__global__ void flow_finish(const float* __restrict__ GC, const double* __restrict__ VX,
+                            const double* __restrict__ logdet, const int* __restrict__ idx, const int* __restrict__ n_dev,
+                            int n, int D, double* __restrict__ G, int64_t ldg, double* __restrict__ V) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= rows_of(n_dev, n)) return;
+  const int64_t c = idx ? idx[r] : r;
+  G[c * ldg + j] = (double)GC[i];
+  if (j == 0) V[c] = VX[r] - logdet[r];
+}
+
+// z-side rows to FP64
+__global__ void flow_rows_out(const float* __restrict__ U, int64_t n, double* __restrict__ Z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Z[i] = (double)U[i];
+}
+
+// column sums (bias gradients): out[c] = sum_r A[r][c]
+__global__ void flow_colsum(const float* __restrict__ A, int rows, int cols, float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.0f;
+  for (int r = 0; r < rows; ++r) s += A[(int64_t)r * cols + c];
+  out[c] = s;
+}
+
+// per-row 0.5|z|^2
+__global__ void flow_half_sq(const float* __restrict__ Z, int D, int n, double* __restrict__ out) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (r >= n) return;
+  double acc = 0;
+  for (int j = lane; j < D; j += NUTS_WAVE) {
+    const double z = Z[(int64_t)r * D + j];
+    acc += z * z;
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) out[r] = 0.5 * acc;
+}
+
+__global__ void flow_loss_rows(double* __restrict__ half_sq, const double* __restrict__ logdet_inv, int n) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) half_sq[r] -= logdet_inv[r];
+}
+
+__global__ void flow_scale(const float* __restrict__ A, int64_t n, float f, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = A[i] * f;
+}
+
+// minibatch rows: floor(u * n_draws) from a Philox stream keyed by (seed, step)
+__global__ void flow_pick_rows(uint64_t seed, uint32_t step, int n, int64_t n_draws, int64_t* __restrict__ pick) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double u = stream_uniform(seed, step, 3u, (uint64_t)r);
+  int64_t k = (int64_t)(u * (double)n_draws);
+  pick[r] = k < n_draws ? k : n_draws - 1;
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1), step t >= 1
+__global__ void flow_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                          float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+static dim3 grid1(int64_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+
+#define HIPF(x)                                                                         \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+static T* dalloc(int64_t n) {
+  T* p = nullptr;
+  if (n > 0) HIPF(hipMalloc(&p, n * sizeof(T)));
+  return p;
+}
+
+// ------------------------------------------------------------------- masks
+FlowSpec flow_default_spec(int D, int L, int H, int n_glob, int group_size) {
+  FlowSpec s;
+  s.D = D; s.L = L; s.H = H;
+  std::vector<int> glob, even, odd, loc;
+  for (int i = 0; i < n_glob && i < D; ++i) glob.push_back(i);
+  for (int i = n_glob; i < D; ++i) {
+    loc.push_back(i);
+    (((i - n_glob) / group_size) % 2 == 0 ? even : odd).push_back(i);
+  }
+  if (odd.empty()) {
+    const size_t h = loc.size() / 2;
+    even.assign(loc.begin(), loc.begin() + h);
+    odd.assign(loc.begin() + h, loc.end());
+  }
+  auto cat = [](const std::vector<int>& a, const std::vector<int>& b) {
+    std::vector<int> r(a);
+    r.insert(r.end(), b.begin(), b.end());
+    return r;
+  };
+  for (int l = 0; l < L; ++l) {
+    std::vector<int> a, b;
+    if (!glob.empty()) {
+      const int k = l % 3;
+      if (k == 0) { a = cat(glob, even); b = odd; }
+      else if (k == 1) { a = cat(glob, odd); b = even; }
+      else { a = loc; b = glob; }
+    } else {
+      a = (l % 2 == 0) ? even : odd;
+      b = (l % 2 == 0) ? odd : even;
+    }
+    s.ia.push_back(a);
+    s.ib.push_back(b);
+  }
+  return s;
+}
+
+// ------------------------------------------------------------------ FlowNet
+FlowNet::FlowNet(const FlowSpec& spec, int max_rows, hipStream_t st) : spec_(spec), R_(max_rows), st_(st) {
+  const int D = spec.D, L = spec.L, H = spec.H;
+  if (D < 2 || L < 1 || H < 1 || max_rows < 1 || (int)spec.ia.size() != L || (int)spec.ib.size() != L)
+    throw std::invalid_argument("flow: bad specification");
+  int64_t off = 0;
+  for (int l = 0; l < L; ++l) {
+    Layer ly{};
+    ly.Da = (int)spec.ia[l].size();
+    ly.Db = (int)spec.ib[l].size();
+    if (ly.Da < 1 || ly.Db < 1 || ly.Da + ly.Db != D) throw std::invalid_argument("flow: masks must partition 0..D-1");
+    std::vector<int> seen(D, 0);
+    for (int v : spec.ia[l]) { if (v < 0 || v >= D || seen[v]++) throw std::invalid_argument("flow: bad mask"); }
+    for (int v : spec.ib[l]) { if (v < 0 || v >= D || seen[v]++) throw std::invalid_argument("flow: bad mask"); }
+    const int64_t sz[6] = {(int64_t)ly.Da * H, H, (int64_t)H * H, H, (int64_t)H * 2 * ly.Db, 2 * (int64_t)ly.Db};
+    for (int k = 0; k < 6; ++k) { ly.off[k] = off; off += sz[k]; }
+    ly.ia = dalloc<int>(ly.Da);
+    ly.ib = dalloc<int>(ly.Db);
+    HIPF(hipMemcpy(ly.ia, spec.ia[l].data(), ly.Da * sizeof(int), hipMemcpyHostToDevice));
+    HIPF(hipMemcpy(ly.ib, spec.ib[l].data(), ly.Db * sizeof(int), hipMemcpyHostToDevice));
+    Dmax_a_ = std::max(Dmax_a_, ly.Da);
+    Dmax_b_ = std::max(Dmax_b_, ly.Db);
+    layers_.push_back(ly);
+  }
+  n_params_ = off;
+  params_ = dalloc<float>(off); grad_ = dalloc<float>(off);
+  adam_m_ = dalloc<float>(off); adam_v_ = dalloc<float>(off);
+  loc_ = dalloc<double>(D); scale_ = dalloc<double>(D);
+  const int64_t R = R_;
+  U_ = dalloc<float>((L + 1) * R * D);
+  UA_ = dalloc<float>(R * Dmax_a_);
+  H1_ = dalloc<float>(L * R * H); H2_ = dalloc<float>(L * R * H);
+  O_ = dalloc<float>(R * 2 * Dmax_b_);
+  S_ = dalloc<float>(L * R * Dmax_b_);
+  GO_ = dalloc<float>(R * 2 * Dmax_b_);
+  GP2_ = dalloc<float>(R * H); GP1_ = dalloc<float>(R * H);
+  GUA_ = dalloc<float>(R * Dmax_a_);
+  GC_ = dalloc<float>(R * D);
+  rowbuf_ = dalloc<double>(R);
+  rowbuf2_ = dalloc<double>(R);
+  pick_ = dalloc<int64_t>(R);
+  // split-K partials: splits * M * N <= (512 + tiles) * 64 * 64 for any GEMM here
+  part_n_ = (int64_t)1024 * FG_BM * FG_BN;
+  part_ = dalloc<float>(part_n_);
+  std::vector<int> dbs(L);
+  for (int l = 0; l < L; ++l) dbs[l] = layers_[l].Db;
+  db_dev_ = dalloc<int>(L);
+  HIPF(hipMemcpy(db_dev_, dbs.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<double> z(D, 0.0), o(D, 1.0);
+  set_standardization(z.data(), o.data());
+  init_params(1);
+}
+
+FlowNet::~FlowNet() {
+  for (auto& l : layers_) { (void)hipFree(l.ia); (void)hipFree(l.ib); }
+  for (void* p : {(void*)params_, (void*)grad_, (void*)adam_m_, (void*)adam_v_, (void*)loc_, (void*)scale_, (void*)U_,
+                  (void*)UA_, (void*)H1_, (void*)H2_, (void*)O_, (void*)S_, (void*)GO_, (void*)GP2_, (void*)GP1_,
+                  (void*)GUA_, (void*)GC_, (void*)part_, (void*)rowbuf_, (void*)rowbuf2_, (void*)pick_,
+                  (void*)db_dev_})
+    if (p) (void)hipFree(p);
+}
+
+void FlowNet::set_params(const float* host) {
+  HIPF(hipMemcpyAsync(params_, host, n_params_ * sizeof(float), hipMemcpyHostToDevice, st_));
+  HIPF(hipStreamSynchronize(st_));
+}
+
+void FlowNet::get_params(float* host) const {
+  HIPF(hipMemcpyAsync(host, params_, n_params_ * sizeof(float), hipMemcpyDeviceToHost, st_));
+  HIPF(hipStreamSynchronize(st_));
+}
+
+void FlowNet::init_params(uint64_t seed) {
+  std::vector<float> p(n_params_, 0.0f);
+  std::mt19937_64 rng(seed);
+  const int H = spec_.H;
+  for (auto& ly : layers_) {
+    const int fan[2][2] = {{ly.Da, H}, {H, H}};
+    const int64_t o[2] = {ly.off[0], ly.off[2]};
+    for (int k = 0; k < 2; ++k) {
+      const double lim = std::sqrt(6.0 / (fan[k][0] + fan[k][1]));
+      std::uniform_real_distribution<double> U(-lim, lim);
+      for (int64_t i = 0; i < (int64_t)fan[k][0] * fan[k][1]; ++i) p[o[k] + i] = (float)U(rng);
+    }
+  }
+  set_params(p.data());
+  HIPF(hipMemsetAsync(adam_m_, 0, n_params_ * sizeof(float), st_));
+  HIPF(hipMemsetAsync(adam_v_, 0, n_params_ * sizeof(float), st_));
+  HIPF(hipStreamSynchronize(st_));
+  adam_t_ = 0;
+}
+
+void FlowNet::set_standardization(const double* loc, const double* scale) {
+  const int D = spec_.D;
+  log_scale_sum_ = 0;
+  for (int j = 0; j < D; ++j) {
+    if (!(scale[j] > 0) || !std::isfinite(scale[j]) || !std::isfinite(loc[j]))
+      throw std::invalid_argument("flow: standardization must be finite with positive scale");
+    log_scale_sum_ += std::log(scale[j]);
+  }
+  HIPF(hipMemcpyAsync(loc_, loc, D * sizeof(double), hipMemcpyHostToDevice, st_));
+  HIPF(hipMemcpyAsync(scale_, scale, D * sizeof(double), hipMemcpyHostToDevice, st_));
+  HIPF(hipStreamSynchronize(st_));
+}
+
+void FlowNet::gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn, float* C,
+                   int64_t ldc, int M, int N, int K, const int* Mdev, int epi, const float* bias, const float* aux,
+                   int64_t ldaux) {
+  GemmDesc g{};
+  g.A = A; g.sam = sam; g.sak = sak; g.B = B; g.sbk = sbk; g.sbn = sbn; g.C = C; g.ldc = ldc;
+  g.M = M; g.N = N; g.K = K; g.Mdev = Mdev; g.epi = epi; g.bias = bias; g.aux = aux; g.ldaux = ldaux;
+  const int tm = (M + FG_BM - 1) / FG_BM, tn = (N + FG_BN - 1) / FG_BN;
+  const int kblocks = (K + FG_BK - 1) / FG_BK;
+  // split K until about 512 blocks, keeping >= 4 K-tiles per split
+  int splits = std::max(1, std::min(kblocks / 4, (512 + tm * tn - 1) / (tm * tn)));
+  g.kper = ((kblocks + splits - 1) / splits) * FG_BK;
+  splits = (K + g.kper - 1) / g.kper;
+  g.splits = splits;
+  if (splits > 1) {
+    if ((int64_t)splits * M * N > part_n_) throw std::logic_error("flow: split-K workspace too small");
+    g.part = part_;
+  }
+  hipLaunchKernelGGL(flow_gemm_kernel, dim3(tn, tm, splits), dim3(256), 0, st_, g);
+  if (splits > 1) hipLaunchKernelGGL(flow_gemm_reduce, grid1((int64_t)M * N), dim3(256), 0, st_, g);
+}
+
+// MLP of layer l on UA (rows x Da): H1, H2 of the layer, O = [s_raw, t]
+void FlowNet::mlp(int l, const float* UA, const int* n_dev, int rows) {
+  const Layer& ly = layers_[l];
+  const int H = spec_.H;
+  const float* P = params_;
+  float* h1 = H1_ + (int64_t)l * R_ * H;
+  float* h2 = H2_ + (int64_t)l * R_ * H;
+  gemm(UA, ly.Da, 1, P + ly.off[0], H, 1, h1, H, rows, H, ly.Da, n_dev, FG_EPI_BIAS_TANH, P + ly.off[1], nullptr, 0);
+  gemm(h1, H, 1, P + ly.off[2], H, 1, h2, H, rows, H, H, n_dev, FG_EPI_BIAS_TANH, P + ly.off[3], nullptr, 0);
+  gemm(h2, H, 1, P + ly.off[4], 2 * ly.Db, 1, O_, 2 * ly.Db, rows, 2 * ly.Db, H, n_dev, FG_EPI_BIAS, P + ly.off[5],
+       nullptr, 0);
+}
+
+void FlowNet::forward(const double* Q, int64_t ldq, const int* idx, const int* n_dev, int n_rows, double* X,
+                      double* logdet) {
+  if (n_rows > R_) throw std::invalid_argument("flow: too many rows");
+  const int D = spec_.D, L = spec_.L;
+  const int64_t RD = (int64_t)R_ * D;
+  hipLaunchKernelGGL(flow_gather_z, grid1((int64_t)n_rows * D), dim3(256), 0, st_, Q, ldq, idx, n_dev, n_rows, D, U_);
+  for (int l = 0; l < L; ++l) {
+    const Layer& ly = layers_[l];
+    float* U = U_ + l * RD;
+    hipLaunchKernelGGL(flow_gather_cols, grid1((int64_t)n_rows * ly.Da), dim3(256), 0, st_, U, D, ly.ia, ly.Da, n_dev,
+                       n_rows, UA_);
+    mlp(l, UA_, n_dev, n_rows);
+    hipLaunchKernelGGL(flow_couple_fwd, grid1((int64_t)n_rows * D), dim3(256), 0, st_, U, O_, ly.ia, ly.ib, ly.Da,
+                       ly.Db, D, n_dev, n_rows, S_ + (int64_t)l * R_ * Dmax_b_, U + RD);
+  }
+  hipLaunchKernelGGL(flow_logdet, dim3((n_rows + 3) / 4), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_, L,
+                     n_dev, n_rows, log_scale_sum_, logdet, 1);
+  hipLaunchKernelGGL(flow_to_x, grid1((int64_t)n_rows * D), dim3(256), 0, st_, U_ + L * RD, loc_, scale_, D, n_dev,
+                     n_rows, X);
+}
+
+void FlowNet::pullback(const double* GX, const double* VX, const double* logdet, const int* idx, const int* n_dev,
+                       int n_rows, double* G, int64_t ldg, double* V) {
+  const int D = spec_.D, L = spec_.L, H = spec_.H;
+  const int64_t RD = (int64_t)R_ * D;
+  hipLaunchKernelGGL(flow_bwd_init, grid1((int64_t)n_rows * D), dim3(256), 0, st_, GX, scale_, D, n_dev, n_rows, GC_);
+  for (int l = L - 1; l >= 0; --l) {
+    const Layer& ly = layers_[l];
+    const float* P = params_;
+    const float* h1 = H1_ + (int64_t)l * R_ * H;
+    const float* h2 = H2_ + (int64_t)l * R_ * H;
+    hipLaunchKernelGGL(flow_couple_bwd_pull, grid1((int64_t)n_rows * ly.Db), dim3(256), 0, st_, GC_, U_ + l * RD,
+                       S_ + (int64_t)l * R_ * Dmax_b_, ly.ib, ly.Db, D, n_dev, n_rows, GO_);
+    // GP2 = (GO W3^T) (1 - h2^2); GP1 = (GP2 W2^T) (1 - h1^2); GUA = GP1 W1^T
+    gemm(GO_, 2 * ly.Db, 1, P + ly.off[4], 1, 2 * ly.Db, GP2_, H, n_rows, H, 2 * ly.Db, n_dev, FG_EPI_DTANH, nullptr,
+         h2, H);
+    gemm(GP2_, H, 1, P + ly.off[2], 1, H, GP1_, H, n_rows, H, H, n_dev, FG_EPI_DTANH, nullptr, h1, H);
+    gemm(GP1_, H, 1, P + ly.off[0], 1, H, GUA_, ly.Da, n_rows, ly.Da, H, n_dev, FG_EPI_NONE, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(flow_scatter_add, grid1((int64_t)n_rows * ly.Da), dim3(256), 0, st_, GC_, GUA_, ly.ia, ly.Da, D,
+                       n_dev, n_rows);
+  }
+  hipLaunchKernelGGL(flow_finish, grid1((int64_t)n_rows * D), dim3(256), 0, st_, GC_, VX, logdet, idx, n_dev, n_rows,
+                     D, G, ldg, V);
+}
+
+// Inverse direction on U_[L] (standardized x, rows): U_[l] = f_l^{-1}(U_[l+1]),
+// keeping H1, H2, S of every layer for backward_params().
+void FlowNet::inverse_pass(int rows) {
+  const int D = spec_.D, L = spec_.L;
+  const int64_t RD = (int64_t)R_ * D;
+  for (int l = L - 1; l >= 0; --l) {
+    const Layer& ly = layers_[l];
+    const float* Vin = U_ + (l + 1) * RD;
+    hipLaunchKernelGGL(flow_gather_cols, grid1((int64_t)rows * ly.Da), dim3(256), 0, st_, Vin, D, ly.ia, ly.Da,
+                       nullptr, rows, UA_);
+    mlp(l, UA_, nullptr, rows);
+    hipLaunchKernelGGL(flow_couple_inv, grid1((int64_t)rows * D), dim3(256), 0, st_, Vin, O_, ly.ia, ly.ib, ly.Da,
+                       ly.Db, D, rows, S_ + (int64_t)l * R_ * Dmax_b_, U_ + l * RD);
+  }
+}
+
+// Row losses 0.5|z|^2 - log|det dT^{-1}/dx| into rowbuf_ (rows) after inverse_pass().
+void FlowNet::row_losses(int rows) {
+  hipLaunchKernelGGL(flow_logdet, dim3((rows + 3) / 4), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_,
+                     spec_.L, nullptr, rows, log_scale_sum_, rowbuf2_, -1);
+  hipLaunchKernelGGL(flow_half_sq, dim3((rows + 3) / 4), dim3(256), 0, st_, U_, spec_.D, rows, rowbuf_);
+  hipLaunchKernelGGL(flow_loss_rows, grid1(rows), dim3(256), 0, st_, rowbuf_, rowbuf2_, rows);
+}
+
+// Gradient of the mean row loss with respect to every parameter, into grad_.
+void FlowNet::backward_params(int rows) {
+  const int D = spec_.D, L = spec_.L, H = spec_.H;
+  const int64_t RD = (int64_t)R_ * D;
+  float* G = grad_;
+  const float* P = params_;
+  hipLaunchKernelGGL(flow_scale, grid1((int64_t)rows * D), dim3(256), 0, st_, U_, (int64_t)rows * D, 1.0f / rows, GC_);
+  for (int l = 0; l < L; ++l) {
+    const Layer& ly = layers_[l];
+    const int Db2 = 2 * ly.Db;
+    const float* h1 = H1_ + (int64_t)l * R_ * H;
+    const float* h2 = H2_ + (int64_t)l * R_ * H;
+    hipLaunchKernelGGL(flow_couple_bwd_train, grid1((int64_t)rows * ly.Db), dim3(256), 0, st_, GC_, U_ + l * RD,
+                       S_ + (int64_t)l * R_ * Dmax_b_, ly.ib, ly.Db, D, rows, 1.0f / rows, GO_);
+    // W3, b3
+    gemm(h2, 1, H, GO_, Db2, 1, G + ly.off[4], Db2, H, Db2, rows, nullptr, FG_EPI_NONE, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(flow_colsum, grid1(Db2), dim3(256), 0, st_, GO_, rows, Db2, G + ly.off[5]);
+    // GP2 = (GO W3^T)(1 - h2^2); W2, b2
+    gemm(GO_, Db2, 1, P + ly.off[4], 1, Db2, GP2_, H, rows, H, Db2, nullptr, FG_EPI_DTANH, nullptr, h2, H);
+    gemm(h1, 1, H, GP2_, H, 1, G + ly.off[2], H, H, H, rows, nullptr, FG_EPI_NONE, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(flow_colsum, grid1(H), dim3(256), 0, st_, GP2_, rows, H, G + ly.off[3]);
+    // GP1 = (GP2 W2^T)(1 - h1^2); W1 from the layer's conditioner input, b1
+    gemm(GP2_, H, 1, P + ly.off[2], 1, H, GP1_, H, rows, H, H, nullptr, FG_EPI_DTANH, nullptr, h1, H);
+    hipLaunchKernelGGL(flow_gather_cols, grid1((int64_t)rows * ly.Da), dim3(256), 0, st_, U_ + (l + 1) * RD, D, ly.ia,
+                       ly.Da, nullptr, rows, UA_);
+    gemm(UA_, 1, ly.Da, GP1_, H, 1, G + ly.off[0], H, ly.Da, H, rows, nullptr, FG_EPI_NONE, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(flow_colsum, grid1(H), dim3(256), 0, st_, GP1_, rows, H, G + ly.off[1]);
+    // cotangent of the conditioner input
+    gemm(GP1_, H, 1, P + ly.off[0], 1, H, GUA_, ly.Da, rows, ly.Da, H, nullptr, FG_EPI_NONE, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(flow_scatter_add, grid1((int64_t)rows * ly.Da), dim3(256), 0, st_, GC_, GUA_, ly.ia, ly.Da, D,
+                       nullptr, rows);
+  }
+}
+
+void FlowNet::inverse(const double* X, int n_rows, double* Z, double* logdet) {
+  if (n_rows > R_) throw std::invalid_argument("flow: too many rows");
+  const int D = spec_.D, L = spec_.L;
+  hipLaunchKernelGGL(flow_standardize, grid1((int64_t)n_rows * D), dim3(256), 0, st_, X, nullptr, loc_, scale_, D,
+                     n_rows, U_ + (int64_t)L * R_ * D);
+  inverse_pass(n_rows);
+  hipLaunchKernelGGL(flow_rows_out, grid1((int64_t)n_rows * D), dim3(256), 0, st_, U_, (int64_t)n_rows * D, Z);
+  if (logdet)
+    hipLaunchKernelGGL(flow_logdet, dim3((n_rows + 3) / 4), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_, L,
+                       nullptr, n_rows, log_scale_sum_, logdet, -1);
+}
+
+double FlowNet::loss_grad(const double* X, int n_rows, float* grad_host) {
+  if (n_rows > R_) throw std::invalid_argument("flow: too many rows");
+  const int D = spec_.D, L = spec_.L;
+  hipLaunchKernelGGL(flow_standardize, grid1((int64_t)n_rows * D), dim3(256), 0, st_, X, nullptr, loc_, scale_, D,
+                     n_rows, U_ + (int64_t)L * R_ * D);
+  inverse_pass(n_rows);
+  row_losses(n_rows);
+  backward_params(n_rows);
+  std::vector<double> lr(n_rows);
+  HIPF(hipMemcpyAsync(lr.data(), rowbuf_, n_rows * sizeof(double), hipMemcpyDeviceToHost, st_));
+  if (grad_host) HIPF(hipMemcpyAsync(grad_host, grad_, n_params_ * sizeof(float), hipMemcpyDeviceToHost, st_));
+  HIPF(hipStreamSynchronize(st_));
+  double s = 0;
+  for (double v : lr) s += v;
+  return s / n_rows;
+}
+
+double FlowNet::train(const double* draws, int64_t n_draws, int steps, int batch, float lr, uint64_t seed,
+                      int report) {
+  if (batch > R_ || batch < 1 || n_draws < 1) throw std::invalid_argument("flow: bad training batch");
+  const int D = spec_.D, L = spec_.L;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double loss_sum = 0;
+  int loss_n = 0;
+  std::vector<double> lrow(batch);
+  for (int t = 1; t <= steps; ++t) {
+    hipLaunchKernelGGL(flow_pick_rows, grid1(batch), dim3(256), 0, st_, seed, (uint32_t)t, batch, n_draws, pick_);
+    hipLaunchKernelGGL(flow_standardize, grid1((int64_t)batch * D), dim3(256), 0, st_, draws, pick_, loc_, scale_, D,
+                       batch, U_ + (int64_t)L * R_ * D);
+    inverse_pass(batch);
+    backward_params(batch);
+    adam_t_ += 1;
+    const float c1 = 1.0f - std::pow(b1, (float)adam_t_), c2 = 1.0f - std::pow(b2, (float)adam_t_);
+    hipLaunchKernelGGL(flow_adam, grid1(n_params_), dim3(256), 0, st_, params_, grad_, adam_m_, adam_v_, n_params_, lr,
+                       b1, b2, eps, c1, c2);
+    if (t > steps - report) {
+      row_losses(batch);
+      HIPF(hipMemcpyAsync(lrow.data(), rowbuf_, batch * sizeof(double), hipMemcpyDeviceToHost, st_));
+      HIPF(hipStreamSynchronize(st_));
+      for (double v : lrow) loss_sum += v;
+      loss_n += batch;
+    }
+  }
+  HIPF(hipStreamSynchronize(st_));
+  return loss_n ? loss_sum / loss_n : 0.0;
+}
+
+}  // namespace nutsgpu

@@ -188,7 +188,9 @@ int glm_grad_mfma_f32(const GlmArgs& a, const GlmPlan& p, const void* Xlow, int 
   else if (a.D <= 64) GF_LAUNCH(XT, 64);        \
   else if (a.D <= 112) GF_LAUNCH(XT, 112);      \
   else GF_LAUNCH(XT, 128);
+  if (a.ev_begin) (void)hipEventRecord(a.ev_begin, st);
   if (precision == 1) { GF_BUCKETS(float) } else { GF_BUCKETS(unsigned short) }
+  if (a.ev_end) (void)hipEventRecord(a.ev_end, st);
 #undef GF_BUCKETS
 #undef GF_LAUNCH
   glm_reduce_launch(a, p, st);

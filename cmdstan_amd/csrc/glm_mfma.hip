@@ -183,7 +183,7 @@ __global__ void glm_mfma_reduce(const double* __restrict__ Wg, const double* __r
 // Active-chain list for the lockstep gradient: idx[0..n) = chains with need[c]
 // set, in chain order (single-block scan, deterministic); clears need[].
 __global__ void __launch_bounds__(1024) compact_active(int* __restrict__ need, int C, int* __restrict__ idx,
-                                                       int* __restrict__ n_active) {
+                                                       int* __restrict__ n_active, long long* __restrict__ tally) {
   __shared__ int wsum[16];
   __shared__ int base;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -212,11 +212,14 @@ __global__ void __launch_bounds__(1024) compact_active(int* __restrict__ need, i
     }
     __syncthreads();
   }
-  if (t == 0) *n_active = base;
+  if (t == 0) {
+    *n_active = base;
+    if (tally) *tally += base;
+  }
 }
 
-void launch_compact_active(int* need, int C, int* idx, int* n_active, hipStream_t st) {
-  hipLaunchKernelGGL(compact_active, dim3(1), dim3(1024), 0, st, need, C, idx, n_active);
+void launch_compact_active(int* need, int C, int* idx, int* n_active, long long* tally, hipStream_t st) {
+  hipLaunchKernelGGL(compact_active, dim3(1), dim3(1024), 0, st, need, C, idx, n_active, tally);
 }
 
 GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus) {
@@ -246,6 +249,7 @@ int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
 #define GM_LAUNCH(K)                                                                                            \
   hipLaunchKernelGGL(glm_mfma_f64<K>, dim3(nblocks), dim3(256), 0, st, a.X, a.y, a.N, a.D, p.rows_per_chunk,    \
                      p.nchunks, p.ngroups, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp)
+  if (a.ev_begin) (void)hipEventRecord(a.ev_begin, st);
   // operand registers scale with the D bucket
   if (a.D <= 16) GM_LAUNCH(16);
   else if (a.D <= 32) GM_LAUNCH(32);
@@ -253,6 +257,7 @@ int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
   else if (a.D <= 112) GM_LAUNCH(112);
   else GM_LAUNCH(128);
 #undef GM_LAUNCH
+  if (a.ev_end) (void)hipEventRecord(a.ev_end, st);
   glm_reduce_launch(a, p, st);
   return 0;
 }

@@ -51,6 +51,8 @@ struct GlmArgs {
   double* work;
   const int* idx = nullptr;       // optional compacted chain list (positions -> chains)
   const int* n_active = nullptr;  // its length (device)
+  hipEvent_t ev_begin = nullptr;  // optional events recorded around the contraction kernel alone
+  hipEvent_t ev_end = nullptr;
 };
 size_t glm_workspace_doubles(int N, int C, int D, int rows_per_chunk);
 int glm_grad_f64(const GlmArgs& a, hipStream_t st);
@@ -61,7 +63,8 @@ struct GlmPlan {
   size_t work_doubles = 0;
 };
 GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus);
-void launch_compact_active(int* need, int C, int* idx, int* n_active, hipStream_t st);
+// tally (optional, device) accumulates the number of active chains over launches.
+void launch_compact_active(int* need, int C, int* idx, int* n_active, long long* tally, hipStream_t st);
 void glm_reduce_launch(const GlmArgs& a, const GlmPlan& p, hipStream_t st);
 
 // FP32 matrix-core version; X pre-converted to FP32 (precision 1) or BF16

@@ -2,6 +2,7 @@
 // of the per-chain model kernels.  Compiled with -ffp-contract=off so the
 // leapfrog and Hamiltonian arithmetic rounds like the FP64 reference restatement.
 #include "launch.h"
+#include "flow.h"
 #include "models_device.h"
 #include "nuts_device.h"
 
@@ -29,6 +30,19 @@ __global__ void __launch_bounds__(256) model_grad_kernel(F f, int C, const int* 
     V[c] = v;
     if (need) need[c] = 0;
   }
+}
+
+// Model gradient at packed rows (the flow's x = T(z)): row r belongs to chain idx[r].
+template <class F>
+__global__ void __launch_bounds__(256) model_grad_rows_kernel(F f, const int* idx, const int* n_dev, int n, int D,
+                                                              const double* X, double* GX, double* VX) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nr = n_dev ? min(n, *n_dev) : n;
+  if (r >= nr) return;
+  const int c = idx ? idx[r] : r;
+  const double v = f(c, X + (int64_t)r * D, GX + (int64_t)r * D, D, lane);
+  if (lane == 0) VX[r] = v;
 }
 
 template <class F>
@@ -86,6 +100,14 @@ void launch_model_grad(int kind, int C, const int* D, const int64_t* off, const 
   const dim3 grid(blocks_for(C)), blk(256);
   NUTS_MODEL_SWITCH(kind, md,
                     hipLaunchKernelGGL((model_grad_kernel<decltype(f)>), grid, blk, 0, st, f, C, D, off, Q, G, V, need));
+}
+
+void launch_model_grad_rows(int kind, const int* idx, const int* n_dev, int n_rows, int D, const double* X,
+                            double* GX, double* VX, const ModelDataDev& md, hipStream_t st) {
+  const dim3 grid(blocks_for(n_rows)), blk(256);
+  NUTS_MODEL_SWITCH(kind, md,
+                    hipLaunchKernelGGL((model_grad_rows_kernel<decltype(f)>), grid, blk, 0, st, f, idx, n_dev, n_rows,
+                                       D, X, GX, VX));
 }
 
 void launch_write_array(int kind, int C, int rows, const int* D, const int64_t* in_off, const int64_t* out_off,

@@ -188,7 +188,8 @@ class Model:
         fit = Fit(draws=per_chain, stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
                   status=status, warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds,
                   leaf_steps=out.leaf_steps, grad_seconds_per_launch=out.grad_seconds_per_launch,
-                  grad_launches_timed=out.grad_launches_timed, column_names=names,
+                  grad_launches_timed=out.grad_launches_timed,
+                  grad_chains_per_launch=out.grad_chains_per_launch, column_names=names,
                   dataset_of_chain=ds_of_chain, config=cfg, error=err)
         if rc != 0:
             raise SamplerError(err)

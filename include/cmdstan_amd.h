@@ -137,10 +137,13 @@ typedef struct nuts_output {
   double warmup_seconds;   /* filled by nuts_sample: GPU wall time of the warmup phase */
   double sampling_seconds; /* ... and of the sampling phase */
   int64_t leaf_steps;      /* lockstep gradient launches (0 in fused mode) */
-  double grad_seconds_per_launch; /* lockstep: mean duration of one batched gradient
-                                     launch in the sampling phase, from HIP events on
-                                     the sampler's stream (0 in fused mode) */
+  double grad_seconds_per_launch; /* lockstep: mean duration of the batched gradient's
+                                     contraction kernel over every launch (warmup and
+                                     sampling), from HIP events on the sampler's
+                                     stream (0 in fused mode) */
   int64_t grad_launches_timed;
+  double grad_chains_per_launch;  /* lockstep GLM path: mean number of active chains a
+                                     gradient launch evaluated (0 elsewhere) */
 } nuts_output;
 
 /* Runs warmup then sampling for every chain of every dataset in one batch.

@@ -12,6 +12,7 @@
 #include <hip/hip_bf16.h>
 
 #include "device_common.h"
+#include "glm_split.h"
 #include "launch.h"
 
 namespace nutsgpu {
@@ -34,8 +35,8 @@ __device__ __forceinline__ float to_f32(unsigned short b) { return __uint_as_flo
 
 template <typename XT, int LAYOUT, int KMAX>
 __global__ void __launch_bounds__(256)
-glm_mfma_f32(const XT* __restrict__ X, const double* __restrict__ y, int N, int D, int rows_per_chunk, int nchunks,
-             int ngroups, const double* __restrict__ Q, int64_t ldq, int C, const int* __restrict__ idx,
+glm_mfma_f32(const XT* __restrict__ X, const double* __restrict__ y, int N, int D, int total_blocks,
+             const double* __restrict__ Q, int64_t ldq, int C, const int* __restrict__ idx,
              const int* __restrict__ n_active, double* __restrict__ Wg, double* __restrict__ Wlp) {
   __shared__ float sX[GF_TN * GF_XS];
   __shared__ double sY[GF_TN];
@@ -45,13 +46,14 @@ glm_mfma_f32(const XT* __restrict__ X, const double* __restrict__ y, int N, int 
   const int lr = l & 15;
   const int lk = l >> 4;
 
+  const int n_act = idx ? *n_active : C;
+  const GlmSplit sp = glm_split(n_act, N, total_blocks);
   const int lin = blockIdx.x;
   const int xcd = lin & 7;
   const int j = lin >> 3;
-  const int chunk = xcd + 8 * (j / ngroups);
-  const int group = j % ngroups;
-  const int n_act = idx ? *n_active : C;
-  if (chunk >= nchunks || group * GF_TC >= n_act) return;
+  const int chunk = xcd + 8 * (j / sp.groups);
+  const int group = j % sp.groups;
+  if (chunk >= sp.chunks || group * GF_TC >= n_act) return;
 
   const int c0 = group * GF_TC + 16 * w;
   const int KS = (D + 3) >> 2;
@@ -72,8 +74,8 @@ glm_mfma_f32(const XT* __restrict__ X, const double* __restrict__ y, int N, int 
   for (int t = 0; t < KMAX / 16; ++t) gacc[t] = f4{0, 0, 0, 0};
   double lp = 0.0;
 
-  const int n_begin = chunk * rows_per_chunk;
-  const int n_end = min(N, n_begin + rows_per_chunk);
+  const int n_begin = chunk * sp.rows;
+  const int n_end = min(N, n_begin + sp.rows);
   const int ncols = DT * 16;
 
   for (int nb = n_begin; nb < n_end; nb += GF_TN) {
@@ -142,13 +144,13 @@ glm_mfma_f32(const XT* __restrict__ X, const double* __restrict__ y, int N, int 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int pos = c0 + f32_row<LAYOUT>(l, i);
-        if (pos < n_act && d < D) Wg[((int64_t)chunk * C + pos) * D + d] = (double)gacc[t][i];
+        if (pos < n_act && d < D) Wg[((int64_t)chunk * sp.stride + pos) * D + d] = (double)gacc[t][i];
       }
     }
   }
   lp += __shfl_xor(lp, 16, 64);
   lp += __shfl_xor(lp, 32, 64);
-  if (lk == 0 && c0 + lr < n_act) Wlp[(int64_t)chunk * C + c0 + lr] = lp;
+  if (lk == 0 && c0 + lr < n_act) Wlp[(int64_t)chunk * sp.stride + c0 + lr] = lp;
 }
 
 __global__ void glm_to_f32(const double* __restrict__ X, float* __restrict__ Y, int64_t n) {
@@ -176,12 +178,11 @@ int glm_grad_mfma_f32(const GlmArgs& a, const GlmPlan& p, const void* Xlow, int 
                       hipStream_t st) {
   if (a.D > GF_KMAX || a.D < 1) return 1;
   double* Wg = a.work;
-  double* Wlp = a.work + (size_t)p.nchunks * a.C * a.D;
-  const int nblocks = (p.nchunks / 8) * p.ngroups * 8;
+  double* Wlp = a.work + (size_t)p.total_blocks * GF_TC * a.D;
   (void)layout;  // the 16x16 f32 result layout measured on gfx950 is layout 0
 #define GF_LAUNCH(XT, K)                                                                                         \
-  hipLaunchKernelGGL((glm_mfma_f32<XT, 0, K>), dim3(nblocks), dim3(256), 0, st, (const XT*)Xlow, a.y, a.N, a.D,  \
-                     p.rows_per_chunk, p.nchunks, p.ngroups, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp)
+  hipLaunchKernelGGL((glm_mfma_f32<XT, 0, K>), dim3(p.total_blocks), dim3(256), 0, st, (const XT*)Xlow, a.y, a.N,   \
+                     a.D, p.total_blocks, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp)
 #define GF_BUCKETS(XT)                          \
   if (a.D <= 16) GF_LAUNCH(XT, 16);             \
   else if (a.D <= 32) GF_LAUNCH(XT, 32);        \

@@ -16,6 +16,7 @@
 // and share its L2.  Chunk partials go to a workspace reduced in fixed order
 // (deterministic, no atomics).
 #include "device_common.h"
+#include "glm_split.h"
 #include "launch.h"
 
 namespace nutsgpu {
@@ -29,8 +30,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 template <int KMAX>
 __global__ void __launch_bounds__(256, 2)
-glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, int D, int rows_per_chunk,
-             int nchunks, int ngroups, const double* __restrict__ Q, int64_t ldq, int C, const int* __restrict__ idx,
+glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, int D, int total_blocks,
+             const double* __restrict__ Q, int64_t ldq, int C, const int* __restrict__ idx,
              const int* __restrict__ n_active, double* __restrict__ Wg, double* __restrict__ Wlp) {
   __shared__ double sX[GM_TN * GM_XS];
   __shared__ double sY[GM_TN];
@@ -41,14 +42,14 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
   const int lk = l >> 4;   // k index inside a 4-deep step
 
   // XCD-aware mapping: consecutive linear ids go to consecutive XCDs.
+  const int n_act = idx ? *n_active : C;
+  const GlmSplit sp = glm_split(n_act, N, total_blocks);
   const int lin = blockIdx.x;
   const int xcd = lin & 7;
   const int j = lin >> 3;
-  const int chunk = xcd + 8 * (j / ngroups);
-  const int group = j % ngroups;
-  // chains are processed in compacted order: position p < n_act maps to chain idx[p]
-  const int n_act = idx ? *n_active : C;
-  if (chunk >= nchunks || group * GM_TC >= n_act) return;
+  const int chunk = xcd + 8 * (j / sp.groups);
+  const int group = j % sp.groups;
+  if (chunk >= sp.chunks || group * GM_TC >= n_act) return;
 
   const int c0 = group * GM_TC + 16 * w;  // this wave's 16 positions
   const int KS = (D + 3) >> 2;            // phase-1 k-steps
@@ -70,8 +71,8 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
   for (int t = 0; t < KMAX / 16; ++t) gacc[t] = d4{0, 0, 0, 0};
   double lp = 0.0;
 
-  const int n_begin = chunk * rows_per_chunk;
-  const int n_end = min(N, n_begin + rows_per_chunk);
+  const int n_begin = chunk * sp.rows;
+  const int n_end = min(N, n_begin + sp.rows);
   const int ncols = DT * 16;
 
   for (int nb = n_begin; nb < n_end; nb += GM_TN) {
@@ -145,34 +146,35 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int pos = c0 + 4 * i + lk;
-        if (pos < n_act && d < D) Wg[((int64_t)chunk * C + pos) * D + d] = gacc[t][i];
+        if (pos < n_act && d < D) Wg[((int64_t)chunk * sp.stride + pos) * D + d] = gacc[t][i];
       }
     }
   }
   // lp: lanes l, l+16, l+32, l+48 hold the same chain
   lp += __shfl_xor(lp, 16, 64);
   lp += __shfl_xor(lp, 32, 64);
-  if (lk == 0 && c0 + lr < n_act) Wlp[(int64_t)chunk * C + c0 + lr] = lp;
+  if (lk == 0 && c0 + lr < n_act) Wlp[(int64_t)chunk * sp.stride + c0 + lr] = lp;
 }
 
-__global__ void glm_mfma_reduce(const double* __restrict__ Wg, const double* __restrict__ Wlp, int nchunks, int C,
-                                int D, double s2inv, const double* __restrict__ Q, int64_t ldq,
-                                const int* __restrict__ idx, const int* __restrict__ n_active,
+__global__ void glm_mfma_reduce(const double* __restrict__ Wg, const double* __restrict__ Wlp, int N,
+                                int total_blocks, int C, int D, double s2inv, const double* __restrict__ Q,
+                                int64_t ldq, const int* __restrict__ idx, const int* __restrict__ n_active,
                                 double* __restrict__ G, double* __restrict__ V) {
   const int pos = blockIdx.x;
   const int n_act = idx ? *n_active : C;
   if (pos >= n_act) return;
+  const GlmSplit sp = glm_split(n_act, N, total_blocks);
   const int c = idx ? idx[pos] : pos;
   const double* q = Q + (int64_t)c * ldq;
   double* g = G + (int64_t)c * ldq;
   for (int k = threadIdx.x; k < D; k += blockDim.x) {
     double s = 0.0;
-    for (int ch = 0; ch < nchunks; ++ch) s += Wg[((int64_t)ch * C + pos) * D + k];
+    for (int ch = 0; ch < sp.chunks; ++ch) s += Wg[((int64_t)ch * sp.stride + pos) * D + k];
     g[k] = s + s2inv * q[k];
   }
   if (threadIdx.x < 64) {
     double lpv = 0.0, pr = 0.0;
-    for (int ch = threadIdx.x; ch < nchunks; ch += 64) lpv += Wlp[(int64_t)ch * C + pos];
+    for (int ch = threadIdx.x; ch < sp.chunks; ch += 64) lpv += Wlp[(int64_t)ch * sp.stride + pos];
     for (int k = threadIdx.x; k < D; k += 64) pr += q[k] * q[k];
     lpv = wave_sum(lpv);
     pr = wave_sum(pr);
@@ -224,31 +226,27 @@ void launch_compact_active(int* need, int C, int* idx, int* n_active, long long*
 
 GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus) {
   GlmPlan p;
-  p.ngroups = (C + GM_TC - 1) / GM_TC;
-  // about two resident blocks per CU, chunk count a multiple of the 8 XCDs
-  int target = 2 * (num_cus > 0 ? num_cus : 256);
-  int nch = (target + p.ngroups - 1) / p.ngroups;
-  nch = ((nch + 7) / 8) * 8;
-  const int max_chunks = ((N + GM_TN - 1) / GM_TN + 7) / 8 * 8;
-  if (nch > max_chunks) nch = max_chunks;
-  if (nch < 8) nch = 8;
-  int rpc = (N + nch - 1) / nch;
-  rpc = ((rpc + GM_TN - 1) / GM_TN) * GM_TN;
-  p.rows_per_chunk = rpc;
-  p.nchunks = nch;
-  p.work_doubles = (size_t)nch * C * D + (size_t)nch * C;
-  (void)D;
+  // about two resident blocks per CU, and at least 8 row chunks per chain group
+  const int groups = (C + GM_TC - 1) / GM_TC;
+  int tb = 2 * (num_cus > 0 ? num_cus : 256);
+  if (tb < 8 * groups) tb = 8 * groups;
+  p.total_blocks = (tb + 7) & ~7;
+  const GlmSplit full = glm_split(C, N, p.total_blocks);
+  p.ngroups = full.groups;
+  p.nchunks = full.chunks;
+  p.rows_per_chunk = full.rows;
+  // partials of any launch: chunks * stride <= total_blocks * 64 positions
+  p.work_doubles = (size_t)p.total_blocks * GM_TC * D + (size_t)p.total_blocks * GM_TC;
   return p;
 }
 
 int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
   if (a.D > GM_KMAX || a.D < 1) return 1;
   double* Wg = a.work;
-  double* Wlp = a.work + (size_t)p.nchunks * a.C * a.D;
-  const int nblocks = (p.nchunks / 8) * p.ngroups * 8;
-#define GM_LAUNCH(K)                                                                                            \
-  hipLaunchKernelGGL(glm_mfma_f64<K>, dim3(nblocks), dim3(256), 0, st, a.X, a.y, a.N, a.D, p.rows_per_chunk,    \
-                     p.nchunks, p.ngroups, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp)
+  double* Wlp = a.work + (size_t)p.total_blocks * GM_TC * a.D;
+#define GM_LAUNCH(K)                                                                                             \
+  hipLaunchKernelGGL(glm_mfma_f64<K>, dim3(p.total_blocks), dim3(256), 0, st, a.X, a.y, a.N, a.D, p.total_blocks, \
+                     a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp)
   if (a.ev_begin) (void)hipEventRecord(a.ev_begin, st);
   // operand registers scale with the D bucket
   if (a.D <= 16) GM_LAUNCH(16);
@@ -264,8 +262,9 @@ int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
 
 void glm_reduce_launch(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
   double* Wg = a.work;
-  double* Wlp = a.work + (size_t)p.nchunks * a.C * a.D;
-  hipLaunchKernelGGL(glm_mfma_reduce, dim3(a.C), dim3(128), 0, st, Wg, Wlp, p.nchunks, a.C, a.D, a.s2inv, a.Q, a.ldq,
+  double* Wlp = a.work + (size_t)p.total_blocks * GM_TC * a.D;
+  hipLaunchKernelGGL(glm_mfma_reduce, dim3(a.C), dim3(128), 0, st, Wg, Wlp, a.N, p.total_blocks, a.C, a.D, a.s2inv,
+                     a.Q, a.ldq,
                      a.idx, a.n_active, a.G, a.V);
 }
 

@@ -59,7 +59,8 @@ int glm_grad_f64(const GlmArgs& a, hipStream_t st);
 
 // FP64 matrix-core version (the sampler's default for logistic regression).
 struct GlmPlan {
-  int ngroups = 0, nchunks = 0, rows_per_chunk = 0;
+  int ngroups = 0, nchunks = 0, rows_per_chunk = 0;  // the split with every chain active
+  int total_blocks = 0;                               // grid of every launch (the split adapts to the active count)
   size_t work_doubles = 0;
 };
 GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus);

@@ -4,6 +4,6 @@ The product is the C-ABI library ``libcmdstan_amd.so`` (include/cmdstan_amd.h);
 this package is its Python front end.
 """
 from ._lib import LIB_PATH, lib, last_error  # noqa: F401
-from .sampler import SAMPLER_COLUMNS, Data, Fit, Model, SamplerError, data_from_json  # noqa: F401
+from .sampler import SAMPLER_COLUMNS, Data, Fit, Flow, Model, SamplerError, data_from_json  # noqa: F401
 
 __all__ = ["Model", "Data", "Fit", "SamplerError", "SAMPLER_COLUMNS", "data_from_json", "lib", "LIB_PATH"]

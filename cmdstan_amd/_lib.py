@@ -21,6 +21,8 @@ class nuts_config(C.Structure):
         ("max_depth", C.c_int), ("metric", C.c_int), ("stepsize", C.c_double), ("stepsize_jitter", C.c_double),
         ("num_chains", C.c_int), ("id", C.c_int), ("seed", C.c_uint64), ("init_radius", C.c_double),
         ("fused", C.c_int), ("precision", C.c_int), ("max_saved_params", C.c_int),
+        ("flow", C.c_int), ("flow_layers", C.c_int), ("flow_hidden", C.c_int), ("flow_train_steps", C.c_int),
+        ("flow_batch", C.c_int), ("flow_lr", C.c_double), ("flow_warmup", C.c_int), ("flow_pilot", C.c_int),
     ]
 
 
@@ -30,6 +32,8 @@ class nuts_output(C.Structure):
         ("status", C.POINTER(C.c_int)), ("warmup_seconds", C.c_double), ("sampling_seconds", C.c_double),
         ("leaf_steps", C.c_int64), ("grad_seconds_per_launch", C.c_double), ("grad_launches_timed", C.c_int64),
         ("grad_chains_per_launch", C.c_double),
+        ("flow_used", C.c_int), ("flow_train_seconds", C.c_double), ("flow_loss", C.c_double),
+        ("pilot_ess_per_grad_plain", C.c_double), ("pilot_ess_per_grad_flow", C.c_double),
     ]
 
 
@@ -55,6 +59,17 @@ SIGNATURES = {
     "nuts_transition_replay": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, DP,
                                          DP, DP, C.c_int64, DP, C.c_int64, DP, DP]),
     "nuts_summary": (C.c_int, [DP, C.c_int, C.c_int, C.c_int, C.c_int, DP]),
+    "nuts_flow_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
+    "nuts_flow_destroy": (None, [C.c_void_p]),
+    "nuts_flow_num_params": (C.c_int64, [C.c_void_p]),
+    "nuts_flow_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "nuts_flow_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "nuts_flow_set_standardization": (C.c_int, [C.c_void_p, DP, DP]),
+    "nuts_flow_forward": (C.c_int, [C.c_void_p, DP, C.c_int, DP, DP]),
+    "nuts_flow_inverse": (C.c_int, [C.c_void_p, DP, C.c_int, DP, DP]),
+    "nuts_flow_pullback_grad": (C.c_int, [C.c_void_p, DP, DP, C.c_int, DP]),
+    "nuts_flow_loss_grad": (C.c_int, [C.c_void_p, DP, C.c_int, DP, C.POINTER(C.c_float)]),
+    "nuts_flow_train": (C.c_int, [C.c_void_p, DP, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, DP]),
     "nuts_last_error": (C.c_char_p, []),
     "nuts_version": (C.c_char_p, []),
 }

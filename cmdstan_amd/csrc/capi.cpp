@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -18,6 +19,7 @@
 #include "../../include/cmdstan_amd.h"
 #include "json.h"
 #include "launch.h"
+#include "flow.h"
 #include "nuts_types.h"
 
 using namespace nutsgpu;
@@ -108,6 +110,7 @@ namespace {
 // -------------------------------------------------------------- models
 struct Dataset {
   int D = 0, nout = 0;
+  int n_glob = 0, group_size = 1;  // flow coupling masks: leading global dims, then groups
   std::vector<std::string> names, unames;
   std::vector<double> block;
 };
@@ -196,6 +199,7 @@ Dataset make_dataset(const std::string& fam, const nuts_data& d, int& kind) {
     s.names.insert(s.names.end(), th.begin(), th.end());
     s.unames.insert(s.unames.end(), th.begin(), th.end());
     if (nc) { auto t2 = idx_names("theta", J); s.names.insert(s.names.end(), t2.begin(), t2.end()); }
+    s.n_glob = 2;
     s.block = {(double)J, mu_s, tau_s};
     s.block.insert(s.block.end(), y.begin(), y.end());
     s.block.insert(s.block.end(), sg.begin(), sg.end());
@@ -229,6 +233,8 @@ Dataset make_dataset(const std::string& fam, const nuts_data& d, int& kind) {
       for (int g = 0; g < G; ++g) s.names.push_back("beta." + std::to_string(g + 1) + "." + std::to_string(k + 1));
     for (int g = 0; g < G; ++g)
       for (int k = 0; k < K; ++k) s.unames.push_back("beta." + std::to_string(g + 1) + "." + std::to_string(k + 1));
+    s.n_glob = 2 * K;
+    s.group_size = K;
     s.block = {(double)N, (double)K, (double)G, d.scalar_or("mu_scale", 5.0)};
     s.block.insert(s.block.end(), gstart.begin(), gstart.end());
     for (int n : order) for (int k = 0; k < K; ++k) s.block.push_back(X[(size_t)n * K + k]);
@@ -300,6 +306,8 @@ struct RunSpec {
   int64_t n_z = 0;
   int first_dataset = 0, num_datasets = -1;
   bool constrain = true;
+  FlowNet* flow = nullptr;                      // sample z with x = T(z) (single dataset, lockstep)
+  const double* stepsize_per_chain = nullptr;   // [C] resume with these step sizes (no adaptation)
 };
 
 struct RunResult {
@@ -313,6 +321,7 @@ struct RunResult {
   double grad_s = 0;
   int64_t grad_timed = 0;
   long long grad_chains = 0;  // chains evaluated, summed over GLM gradient launches
+  std::vector<double> z_last;  // [C][D] unconstrained q of each chain's last saved row (before any flow map)
   int rows = 0;
 };
 
@@ -344,6 +353,14 @@ void validate(const nuts_config& c) {
   if (c.num_chains < 1) throw std::invalid_argument("num_chains must be >= 1");
   if (c.init_radius < 0) throw std::invalid_argument("init radius must be >= 0");
   if (c.precision < 0 || c.precision > 2) throw std::invalid_argument("precision must be 0 (fp64), 1 (fp32) or 2 (bf16)");
+  if (c.flow < 0 || c.flow > 1) throw std::invalid_argument("flow must be 0 or 1");
+  if (c.flow) {
+    if (c.flow_layers < 1 || c.flow_hidden < 1 || c.flow_train_steps < 0 || c.flow_batch < 1 || !(c.flow_lr > 0) ||
+        c.flow_warmup < 1 || c.flow_pilot < 4)
+      throw std::invalid_argument("flow: layers, hidden, batch, lr, flow_warmup must be positive and flow_pilot >= 4");
+    if (!c.adapt_engaged) throw std::invalid_argument("flow: adaptation must be engaged");
+    if (c.save_warmup) throw std::invalid_argument("flow: save_warmup is not supported");
+  }
 }
 
 __global__ void resume_kernel(ChainState* st, int C) {
@@ -394,6 +411,11 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   hipStream_t st;
   HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{st};
+  // the flow works on the sampler's stream for this run
+  struct FlowBind {
+    FlowNet* f; hipStream_t old;
+    ~FlowBind() { if (f) f->bind_stream(old); }
+  } fb{R.flow, R.flow ? R.flow->bind_stream(st) : nullptr};
 
   DevBuf<int> dD(C), dDp(C), dneed(C), dstop(1);
   DevBuf<int64_t> doffb(C), dmoff(C), ddoff(C), ddsoff(C), dooff(C);
@@ -462,8 +484,8 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     s.phase = PH_INIT_TRY;
     s.sample_slot = 0;
     s.free_mask = ((nslots >= 32) ? 0xFFFFFFFFu : ((1u << nslots) - 1u)) & ~1u;
-    s.nom_eps = cfg.stepsize;
-    s.eps = cfg.stepsize;
+    s.nom_eps = R.stepsize_per_chain ? R.stepsize_per_chain[c] : cfg.stepsize;
+    s.eps = s.nom_eps;
     s.adapt_flag = cfg.adapt_engaged ? 1 : 0;
     s.sa_mu = std::log(10 * cfg.stepsize);
     s.w_counter = w.counter; s.w_size = w.size; s.w_next = w.next;
@@ -510,8 +532,13 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   ModelDataDev md{M.d_data.p, ddsoff.p, 1.0};
 
   bool fused = cfg.fused == 1;
-  const bool glm_ok = M.glm && nds == 1;
+  const bool glm_ok = M.glm && nds == 1 && !R.flow;
   if (cfg.fused < 0) fused = !glm_ok;
+  if (R.flow) {
+    if (nds != 1 || R.flow->dim() != D[0] || R.flow->max_rows() < C)
+      throw std::invalid_argument("flow: one dataset whose dimension matches the flow, at most max_rows chains");
+    fused = false;
+  }
   if (!fused && M.kind == MK_LOGISTIC && !glm_ok) fused = false;  // per-chain gradient kernel in lockstep
 
   DevBuf<double> work;
@@ -519,6 +546,12 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   const void* xlow = nullptr;
   GlmArgs ga;
   GlmPlan gplan;
+  DevBuf<double> fX, fGX, fVX, fLD;
+  if (R.flow) {
+    act_idx.alloc(C);
+    n_act.alloc(1);
+    fX.alloc((size_t)C * D[0]); fGX.alloc((size_t)C * D[0]); fVX.alloc(C); fLD.alloc(C);
+  }
   if (!fused && glm_ok) {
     act_idx.alloc(C);
     n_act.alloc(1);
@@ -574,6 +607,14 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
             const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
                                                : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
             if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+          } else if (R.flow) {
+            // pulled-back gradient: x = T(z) for the active chains, model at x, VJP to z
+            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+            HIPC(hipEventRecord(ev.a[k], st));
+            R.flow->forward(Q.p, Dp[0], act_idx.p, n_act.p, C, fX.p, fLD.p);
+            launch_model_grad_rows(M.kind, act_idx.p, n_act.p, C, D[0], fX.p, fGX.p, fVX.p, md, st);
+            R.flow->pullback(fGX.p, fVX.p, fLD.p, act_idx.p, n_act.p, C, G.p, Dp[0], V.p);
+            HIPC(hipEventRecord(ev.b[k], st));
           } else {
             HIPC(hipEventRecord(ev.a[k], st));
             launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
@@ -636,6 +677,32 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     HIPC(hipStreamSynchronize(st));
     for (int c = 0; c < C; ++c) for (int i = 0; i < D[c]; ++i) res.minv.push_back(hm[off[c] + i]);
   }
+  if (rows > 0 && (msp < 0 || msp >= D[0])) {
+    bool uni = true;
+    for (int c = 1; c < C; ++c) uni = uni && D[c] == D[0];
+    if (uni) {
+      // last saved row of each chain, unconstrained (the state a later run resumes from)
+      const int ld = 7 + nsave_of(D[0]);
+      std::vector<double> last((size_t)C * ld);
+      for (int c = 0; c < C; ++c)
+        HIPC(hipMemcpyAsync(last.data() + (size_t)c * ld, draws.p + doff[c] + (int64_t)(rows - 1) * ld,
+                            ld * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      res.z_last.resize((size_t)C * D[0]);
+      for (int c = 0; c < C; ++c)
+        for (int i = 0; i < D[0]; ++i) res.z_last[(size_t)c * D[0] + i] = last[(size_t)c * ld + 7 + i];
+    }
+  }
+  if (R.flow) {
+    // saved rows hold z: replace it by x = T(z) and lp__ by log p(x) = lp(z) - log|det dT/dz|
+    if (msp >= 0 && msp < D[0]) throw std::invalid_argument("flow: max_saved_params must keep every dimension");
+    const int64_t nrows = (int64_t)C * rows, ld = 7 + D[0];
+    const int R0 = R.flow->max_rows();
+    for (int64_t k0 = 0; k0 < nrows; k0 += R0) {
+      const int n = (int)std::min<int64_t>(R0, nrows - k0);
+      R.flow->map_rows(draws.p + k0 * ld, ld, n);
+    }
+  }
   if (R.constrain && msp < 0) {
     DevBuf<double> out(std::max<int64_t>(otot, 1));
     DevBuf<int> dn(C);
@@ -654,6 +721,132 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   HIPC(hipStreamSynchronize(st));
   HIPC(hipGetLastError());
   return res;
+}
+
+// min bulk-ESS over the D unconstrained coordinates of rows [r0, r1) of every
+// chain, divided by the leapfrog steps those rows took
+double pilot_ess_per_grad(const RunResult& r, int r0, int r1, int D) {
+  const int C = (int)r.st.size();
+  const int n = r1 - r0;
+  if (n < 4) return 0.0;
+  std::vector<double> dr((size_t)D * C * n);
+  double grads = 0;
+  for (int c = 0; c < C; ++c) {
+    const double* base = r.draws.data() + r.draw_off[c];
+    const int ld = r.cols[c];
+    for (int k = 0; k < n; ++k) {
+      const double* row = base + (int64_t)(r0 + k) * ld;
+      grads += row[4];
+      for (int p = 0; p < D; ++p) dr[((size_t)p * C + c) * n + k] = row[7 + p];
+    }
+  }
+  std::vector<double> st((size_t)D * 10);
+  if (nuts_summary(dr.data(), n, C, D, 16, st.data()) != NUTS_OK) throw std::runtime_error("pilot summary failed");
+  double mn = std::numeric_limits<double>::infinity();
+  for (int p = 0; p < D; ++p) {
+    const double e = st[(size_t)p * 10 + 7];
+    if (std::isfinite(e)) mn = std::min(mn, e);
+  }
+  if (!std::isfinite(mn) || grads <= 0) return 0.0;
+  return mn / grads;
+}
+
+bool all_done(const RunResult& r) {
+  for (const auto& s : r.st) if (s.phase != PH_DONE) return false;
+  return true;
+}
+
+struct FlowReport {
+  int used = 0;
+  double train_s = 0, loss = 0, egp = 0, egf = 0, pre_s = 0;
+  std::vector<int64_t> nlf, ngrad;  // per chain, over every phase
+};
+
+// Plain warmup + pilot, flow training, flow warmup + pilot, then the better
+// sampler continues for num_samples iterations (no adaptation).
+RunResult run_with_flow(ModelImpl& M, const RunSpec& R0, FlowReport& rep) {
+  const nuts_config& cfg = R0.cfg;
+  if (M.ds.size() != 1) throw std::invalid_argument("flow: the model must have one dataset");
+  const Dataset& ds = M.ds[0];
+  const int D = ds.D, C = cfg.num_chains, P = cfg.flow_pilot;
+  if (D < 2) throw std::invalid_argument("flow: the model needs at least 2 dimensions");
+  if (R0.inj_u || R0.inj_z || R0.stepsize_per_chain) throw std::invalid_argument("flow: unsupported run options");
+  auto add_counts = [&](const RunResult& r) {
+    if (rep.nlf.empty()) { rep.nlf.assign(C, 0); rep.ngrad.assign(C, 0); }
+    for (int c = 0; c < C; ++c) { rep.nlf[c] += r.st[c].n_leapfrog_total; rep.ngrad[c] += r.st[c].n_grad; }
+  };
+  // phase A: plain NUTS, adaptive warmup + pilot, every row saved unconstrained
+  RunSpec A = R0;
+  A.cfg.flow = 0; A.cfg.save_warmup = 1; A.cfg.thin = 1; A.cfg.num_samples = P; A.cfg.max_saved_params = -1;
+  A.constrain = false;
+  RunResult ra = run_batch(M, A);
+  add_counts(ra);
+  if (!all_done(ra)) return ra;
+  const int rowsA = cfg.num_warmup + P;
+  rep.egp = pilot_ess_per_grad(ra, cfg.num_warmup, rowsA, D);
+  // training set: second half of warmup and the pilot
+  const int first = cfg.num_warmup / 2;
+  const int64_t ntr = (int64_t)C * (rowsA - first);
+  std::vector<double> tr((size_t)ntr * D);
+  std::vector<double> mean(D, 0.0), m2(D, 0.0);
+  int64_t k = 0;
+  for (int c = 0; c < C; ++c) {
+    const double* base = ra.draws.data() + ra.draw_off[c];
+    for (int r = first; r < rowsA; ++r, ++k) {
+      const double* row = base + (int64_t)r * ra.cols[c] + 7;
+      for (int i = 0; i < D; ++i) {
+        tr[(size_t)k * D + i] = row[i];
+        const double d = row[i] - mean[i];
+        mean[i] += d / (double)(k + 1);
+        m2[i] += d * (row[i] - mean[i]);
+      }
+    }
+  }
+  std::vector<double> sd(D);
+  for (int i = 0; i < D; ++i) sd[i] = std::max(std::sqrt(m2[i] / std::max<int64_t>(1, ntr - 1)), 1e-8);
+  FlowSpec spec = flow_default_spec(D, cfg.flow_layers, cfg.flow_hidden, ds.n_glob, ds.group_size);
+  FlowNet net(spec, std::max(C, cfg.flow_batch));
+  net.init_params(cfg.seed * 0x9E3779B97F4A7C15ull + 17);
+  net.set_standardization(mean.data(), sd.data());
+  {
+    DevBuf<double> dtr((size_t)ntr * D);
+    dtr.upload(tr, net.own_stream());
+    HIPC(hipStreamSynchronize(net.own_stream()));
+    const auto t0 = std::chrono::steady_clock::now();
+    rep.loss = net.train(dtr.p, ntr, cfg.flow_train_steps, cfg.flow_batch, (float)cfg.flow_lr,
+                         cfg.seed + 0x5851F42D4C957F2Dull, std::min(20, std::max(1, cfg.flow_train_steps)));
+    rep.train_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  // phase B: flow NUTS from z0 = T^{-1}(last pilot draw), adaptive warmup + pilot
+  std::vector<double> z0((size_t)C * D);
+  {
+    DevBuf<double> dx((size_t)C * D), dz((size_t)C * D);
+    dx.upload(ra.z_last, net.own_stream());
+    net.inverse(dx.p, C, dz.p, nullptr);
+    HIPC(hipMemcpyAsync(z0.data(), dz.p, z0.size() * sizeof(double), hipMemcpyDeviceToHost, net.own_stream()));
+    HIPC(hipStreamSynchronize(net.own_stream()));
+  }
+  RunSpec Bs = R0;
+  Bs.cfg.flow = 0; Bs.cfg.num_warmup = cfg.flow_warmup; Bs.cfg.num_samples = P; Bs.cfg.save_warmup = 0;
+  Bs.cfg.thin = 1; Bs.cfg.max_saved_params = -1;
+  Bs.init = z0.data(); Bs.inv_metric = nullptr; Bs.flow = &net; Bs.constrain = false;
+  RunResult rb = run_batch(M, Bs);
+  add_counts(rb);
+  const bool flow_ok = all_done(rb);
+  rep.egf = flow_ok ? pilot_ess_per_grad(rb, 0, P, D) : 0.0;
+  rep.used = flow_ok && rep.egf > rep.egp;
+  rep.pre_s = ra.warm_s + ra.samp_s + rep.train_s + rb.warm_s + rb.samp_s;
+  // phase C: the better sampler continues without adaptation
+  const RunResult& src = rep.used ? rb : ra;
+  std::vector<double> eps(C);
+  for (int c = 0; c < C; ++c) eps[c] = src.st[c].nom_eps;
+  RunSpec Cs = R0;
+  Cs.cfg.flow = 0; Cs.cfg.num_warmup = 0; Cs.cfg.adapt_engaged = 0; Cs.cfg.save_warmup = 0;
+  Cs.init = src.z_last.data(); Cs.inv_metric = src.minv.data(); Cs.stepsize_per_chain = eps.data();
+  Cs.flow = rep.used ? &net : nullptr;
+  RunResult rc = run_batch(M, Cs);
+  add_counts(rc);
+  return rc;
 }
 
 const char* failure_text(int f) {
@@ -853,6 +1046,8 @@ void nuts_config_default(nuts_config* c) {
   c->max_depth = 10; c->metric = NUTS_METRIC_DIAG_E; c->stepsize = 1; c->stepsize_jitter = 0;
   c->num_chains = 1; c->id = 1; c->seed = 0; c->init_radius = 2; c->fused = -1; c->precision = 0;
   c->max_saved_params = -1;
+  c->flow = 0; c->flow_layers = 8; c->flow_hidden = 128; c->flow_train_steps = 1000; c->flow_batch = 256;
+  c->flow_lr = 1e-3; c->flow_warmup = 300; c->flow_pilot = 100;
 }
 
 int nuts_config_validate(const nuts_config* cfg) {
@@ -873,7 +1068,8 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
       for (auto& d : m->impl.ds)
         if (d.D != m->impl.ds[0].D) throw std::invalid_argument("init/inv_metric arrays need equal dimensions");
     }
-    RunResult r = run_batch(m->impl, R);
+    FlowReport rep;
+    RunResult r = cfg->flow ? run_with_flow(m->impl, R, rep) : run_batch(m->impl, R);
     const int C = (int)r.st.size();
     if (out) {
       if (out->draws) std::memcpy(out->draws, r.draws.data(), r.draws.size() * sizeof(double));
@@ -881,8 +1077,8 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
       for (int c = 0; c < C; ++c) {
         const ChainState& s = r.st[c];
         if (out->stepsize) out->stepsize[c] = s.nom_eps;
-        if (out->n_leapfrog) out->n_leapfrog[c] = s.n_leapfrog_total;
-        if (out->grad_evals) out->grad_evals[c] = s.n_grad;
+        if (out->n_leapfrog) out->n_leapfrog[c] = cfg->flow ? rep.nlf[c] : s.n_leapfrog_total;
+        if (out->grad_evals) out->grad_evals[c] = cfg->flow ? rep.ngrad[c] : s.n_grad;
         if (out->status) out->status[c] = s.phase == PH_DONE ? 0 : (s.failure ? s.failure : -1);
       }
       if (out->inv_metric) std::memcpy(out->inv_metric, r.minv.data(), r.minv.size() * sizeof(double));
@@ -893,6 +1089,14 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
       out->grad_seconds_per_launch = r.grad_timed ? r.grad_s / r.grad_timed : 0.0;
       out->grad_launches_timed = r.grad_timed;
       out->grad_chains_per_launch = r.leaf_steps ? (double)r.grad_chains / r.leaf_steps : 0.0;
+      if (cfg->flow) {
+        out->warmup_seconds = rep.pre_s;  // plain warmup + pilot, training, flow warmup + pilot
+        out->flow_used = rep.used;
+        out->flow_train_seconds = rep.train_s;
+        out->flow_loss = rep.loss;
+        out->pilot_ess_per_grad_plain = rep.egp;
+        out->pilot_ess_per_grad_flow = rep.egf;
+      }
     }
     for (int c = 0; c < C; ++c)
       if (r.st[c].phase != PH_DONE) {
@@ -927,6 +1131,122 @@ int nuts_transition_replay(nuts_model* m, int dataset, int num_chains, int metri
       inf[0] = row[3]; inf[1] = row[4]; inf[2] = row[5]; inf[3] = row[1]; inf[4] = row[6]; inf[5] = row[0];
       inf[6] = (double)s.uctr; inf[7] = (double)s.nctr; inf[8] = row[2];
     }
+    return NUTS_OK;
+  });
+}
+
+// ------------------------------------------------------------------ flow
+struct nuts_flow {
+  std::unique_ptr<FlowNet> net;
+};
+
+nuts_flow* nuts_flow_create(int D, int n_layers, int hidden, int n_glob, int group_size, int max_rows,
+                            uint64_t seed) {
+  try {
+    if (n_glob < 0 || group_size < 1 || n_glob >= D) throw std::invalid_argument("flow: bad n_glob / group_size");
+    auto f = std::make_unique<nuts_flow>();
+    f->net = std::make_unique<FlowNet>(flow_default_spec(D, n_layers, hidden, n_glob, group_size), max_rows);
+    f->net->init_params(seed);
+    return f.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void nuts_flow_destroy(nuts_flow* f) { delete f; }
+
+int64_t nuts_flow_num_params(const nuts_flow* f) { return f ? f->net->num_params() : -1; }
+
+int nuts_flow_get_params(const nuts_flow* f, float* out) {
+  return guarded([&]() { f->net->get_params(out); return NUTS_OK; });
+}
+
+int nuts_flow_set_params(nuts_flow* f, const float* in) {
+  return guarded([&]() { f->net->set_params(in); return NUTS_OK; });
+}
+
+int nuts_flow_set_standardization(nuts_flow* f, const double* loc, const double* scale) {
+  return guarded([&]() { f->net->set_standardization(loc, scale); return NUTS_OK; });
+}
+
+static void flow_rows_check(const nuts_flow* f, int n) {
+  if (!f) throw std::invalid_argument("flow: null handle");
+  if (n < 1 || n > f->net->max_rows()) throw std::invalid_argument("flow: n must be in [1, max_rows]");
+}
+
+int nuts_flow_forward(nuts_flow* f, const double* z, int n, double* x, double* logdet) {
+  return guarded([&]() {
+    flow_rows_check(f, n);
+    const int D = f->net->dim();
+    hipStream_t st = f->net->own_stream();
+    DevBuf<double> dz((size_t)n * D), dx((size_t)n * D), dl(n);
+    dz.upload(z, (size_t)n * D, st);
+    f->net->forward(dz.p, D, nullptr, nullptr, n, dx.p, dl.p);
+    HIPC(hipMemcpyAsync(x, dx.p, (size_t)n * D * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(logdet, dl.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return NUTS_OK;
+  });
+}
+
+int nuts_flow_inverse(nuts_flow* f, const double* x, int n, double* z, double* logdet) {
+  return guarded([&]() {
+    flow_rows_check(f, n);
+    const int D = f->net->dim();
+    hipStream_t st = f->net->own_stream();
+    DevBuf<double> dz((size_t)n * D), dx((size_t)n * D), dl(n);
+    dx.upload(x, (size_t)n * D, st);
+    f->net->inverse(dx.p, n, dz.p, dl.p);
+    HIPC(hipMemcpyAsync(z, dz.p, (size_t)n * D * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(logdet, dl.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return NUTS_OK;
+  });
+}
+
+int nuts_flow_pullback_grad(nuts_flow* f, const double* z, const double* gx, int n, double* gz) {
+  return guarded([&]() {
+    flow_rows_check(f, n);
+    const int D = f->net->dim();
+    hipStream_t st = f->net->own_stream();
+    DevBuf<double> dz((size_t)n * D), dx((size_t)n * D), dgx((size_t)n * D), dgz((size_t)n * D), dl(n), dvx(n), dv(n);
+    dz.upload(z, (size_t)n * D, st);
+    dgx.upload(gx, (size_t)n * D, st);
+    HIPC(hipMemsetAsync(dvx.p, 0, n * sizeof(double), st));
+    f->net->forward(dz.p, D, nullptr, nullptr, n, dx.p, dl.p);
+    f->net->pullback(dgx.p, dvx.p, dl.p, nullptr, nullptr, n, dgz.p, D, dv.p);
+    HIPC(hipMemcpyAsync(gz, dgz.p, (size_t)n * D * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return NUTS_OK;
+  });
+}
+
+int nuts_flow_loss_grad(nuts_flow* f, const double* x, int n, double* loss, float* grad) {
+  return guarded([&]() {
+    flow_rows_check(f, n);
+    const int D = f->net->dim();
+    hipStream_t st = f->net->own_stream();
+    DevBuf<double> dx((size_t)n * D);
+    dx.upload(x, (size_t)n * D, st);
+    const double l = f->net->loss_grad(dx.p, n, grad);
+    if (loss) *loss = l;
+    return NUTS_OK;
+  });
+}
+
+int nuts_flow_train(nuts_flow* f, const double* x, int64_t n, int steps, int batch, double lr, uint64_t seed,
+                    double* final_loss) {
+  return guarded([&]() {
+    if (!f || n < 1 || steps < 0 || batch < 1 || batch > f->net->max_rows() || !(lr > 0))
+      throw std::invalid_argument("flow: bad training arguments");
+    const int D = f->net->dim();
+    hipStream_t st = f->net->own_stream();
+    DevBuf<double> dx((size_t)n * D);
+    dx.upload(x, (size_t)n * D, st);
+    HIPC(hipStreamSynchronize(st));
+    const double l = f->net->train(dx.p, n, steps, batch, (float)lr, seed, std::min(20, std::max(1, steps)));
+    if (final_loss) *final_loss = l;
     return NUTS_OK;
   });
 }

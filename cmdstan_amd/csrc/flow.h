@@ -29,7 +29,7 @@ FlowSpec flow_default_spec(int D, int L, int H, int n_glob, int group_size);
 
 class FlowNet {
  public:
-  FlowNet(const FlowSpec& spec, int max_rows, hipStream_t st);
+  FlowNet(const FlowSpec& spec, int max_rows);  // works on a stream of its own until bind_stream()
   ~FlowNet();
   FlowNet(const FlowNet&) = delete;
   FlowNet& operator=(const FlowNet&) = delete;
@@ -38,6 +38,9 @@ class FlowNet {
   int max_rows() const { return R_; }
   int64_t num_params() const { return n_params_; }
   const FlowSpec& spec() const { return spec_; }
+  // Later calls run on s (e.g. the sampler's stream); returns the previous stream.
+  hipStream_t bind_stream(hipStream_t s) { hipStream_t o = st_; st_ = s; return o; }
+  hipStream_t own_stream() const { return own_st_; }
 
   // host <-> device parameters (FP32, layout: per layer W1 [Da][H], b1, W2 [H][H], b2, W3 [H][2Db], b3)
   void set_params(const float* host);
@@ -54,6 +57,9 @@ class FlowNet {
   // writes G[idx[r]] (stride ldg) and V[idx[r]] = VX[r] - logdet[r].
   void pullback(const double* GX, const double* VX, const double* logdet, const int* idx, const int* n_dev,
                 int n_rows, double* G, int64_t ldg, double* V);
+  // In place on n_rows sampler rows (stride ld, z in columns 7..7+D): z -> x = T(z),
+  // column 0 (lp of z) -> lp(z) - log|det dT/dz|.
+  void map_rows(double* rows, int64_t ld, int n_rows);
   // z = T^{-1}(x) for n_rows rows of X (stride D), n_rows <= max_rows.
   void inverse(const double* X, int n_rows, double* Z, double* logdet);
 
@@ -81,7 +87,7 @@ class FlowNet {
   FlowSpec spec_;
   int R_, Dmax_a_ = 0, Dmax_b_ = 0;
   int64_t n_params_ = 0;
-  hipStream_t st_;
+  hipStream_t st_ = nullptr, own_st_ = nullptr;
   std::vector<Layer> layers_;
   float* params_ = nullptr; float* grad_ = nullptr; float* adam_m_ = nullptr; float* adam_v_ = nullptr;
   double* loc_ = nullptr; double* scale_ = nullptr; double log_scale_sum_ = 0;
@@ -96,6 +102,7 @@ class FlowNet {
   float* GC_ = nullptr;    // R x D
   float* part_ = nullptr; int64_t part_n_ = 0;  // split-K partials
   double* rowbuf_ = nullptr; double* rowbuf2_ = nullptr;  // R
+  double* xrows_ = nullptr;   // R x D (map_rows)
   int64_t* pick_ = nullptr;   // R minibatch rows
   int* db_dev_ = nullptr;     // L
   int64_t adam_t_ = 0;

@@ -282,6 +282,16 @@ __global__ void flow_finish(const float* __restrict__ GC, const double* __restri
   if (j == 0) V[c] = VX[r] - logdet[r];
 }
 
+// sampler rows: columns 7.. <- X, column 0 -= logdet
+__global__ void flow_store_rows(const double* __restrict__ X, const double* __restrict__ logdet, int n, int D,
+                                double* __restrict__ rows, int64_t ld) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(i / D), j = (int)(i % D);
+  if (r >= n) return;
+  rows[r * ld + 7 + j] = X[i];
+  if (j == 0) rows[r * ld] -= logdet[r];
+}
+
 // z-side rows to FP64
 __global__ void flow_rows_out(const float* __restrict__ U, int64_t n, double* __restrict__ Z) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,7 +405,9 @@ FlowSpec flow_default_spec(int D, int L, int H, int n_glob, int group_size) {
 }
 
 // ------------------------------------------------------------------ FlowNet
-FlowNet::FlowNet(const FlowSpec& spec, int max_rows, hipStream_t st) : spec_(spec), R_(max_rows), st_(st) {
+FlowNet::FlowNet(const FlowSpec& spec, int max_rows) : spec_(spec), R_(max_rows) {
+  HIPF(hipStreamCreateWithFlags(&own_st_, hipStreamNonBlocking));
+  st_ = own_st_;
   const int D = spec.D, L = spec.L, H = spec.H;
   if (D < 2 || L < 1 || H < 1 || max_rows < 1 || (int)spec.ia.size() != L || (int)spec.ib.size() != L)
     throw std::invalid_argument("flow: bad specification");
@@ -435,6 +447,7 @@ FlowNet::FlowNet(const FlowSpec& spec, int max_rows, hipStream_t st) : spec_(spe
   rowbuf_ = dalloc<double>(R);
   rowbuf2_ = dalloc<double>(R);
   pick_ = dalloc<int64_t>(R);
+  xrows_ = dalloc<double>(R * D);
   // split-K partials: splits * M * N <= (512 + tiles) * 64 * 64 for any GEMM here
   part_n_ = (int64_t)1024 * FG_BM * FG_BN;
   part_ = dalloc<float>(part_n_);
@@ -448,10 +461,14 @@ FlowNet::FlowNet(const FlowSpec& spec, int max_rows, hipStream_t st) : spec_(spe
 }
 
 FlowNet::~FlowNet() {
+  if (own_st_) {
+    (void)hipStreamSynchronize(own_st_);
+    (void)hipStreamDestroy(own_st_);
+  }
   for (auto& l : layers_) { (void)hipFree(l.ia); (void)hipFree(l.ib); }
   for (void* p : {(void*)params_, (void*)grad_, (void*)adam_m_, (void*)adam_v_, (void*)loc_, (void*)scale_, (void*)U_,
                   (void*)UA_, (void*)H1_, (void*)H2_, (void*)O_, (void*)S_, (void*)GO_, (void*)GP2_, (void*)GP1_,
-                  (void*)GUA_, (void*)GC_, (void*)part_, (void*)rowbuf_, (void*)rowbuf2_, (void*)pick_,
+                  (void*)GUA_, (void*)GC_, (void*)part_, (void*)rowbuf_, (void*)rowbuf2_, (void*)pick_, (void*)xrows_,
                   (void*)db_dev_})
     if (p) (void)hipFree(p);
 }
@@ -634,6 +651,13 @@ void FlowNet::backward_params(int rows) {
     hipLaunchKernelGGL(flow_scatter_add, grid1((int64_t)rows * ly.Da), dim3(256), 0, st_, GC_, GUA_, ly.ia, ly.Da, D,
                        nullptr, rows);
   }
+}
+
+void FlowNet::map_rows(double* rows, int64_t ld, int n_rows) {
+  if (n_rows > R_) throw std::invalid_argument("flow: too many rows");
+  forward(rows + 7, ld, nullptr, nullptr, n_rows, xrows_, rowbuf_);
+  hipLaunchKernelGGL(flow_store_rows, grid1((int64_t)n_rows * spec_.D), dim3(256), 0, st_, xrows_, rowbuf_, n_rows,
+                     spec_.D, rows, ld);
 }
 
 void FlowNet::inverse(const double* X, int n_rows, double* Z, double* logdet) {

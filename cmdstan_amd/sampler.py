@@ -125,7 +125,9 @@ class Model:
     def sample(self, num_chains=1, num_samples=1000, num_warmup=1000, save_warmup=False, thin=1,
                adapt_engaged=True, delta=0.8, gamma=0.05, kappa=0.75, t0=10.0, init_buffer=75, term_buffer=50,
                window=25, max_depth=10, metric="diag_e", stepsize=1.0, stepsize_jitter=0.0, id=1, seed=0,
-               init=2.0, inv_metric=None, fused=-1, precision=0, max_saved_params=-1):
+               init=2.0, inv_metric=None, fused=-1, precision=0, max_saved_params=-1, flow=False, flow_layers=8,
+               flow_hidden=128, flow_train_steps=1000, flow_batch=256, flow_lr=1e-3, flow_warmup=300,
+               flow_pilot=100):
         cfg = nuts_config()
         lib().nuts_config_default(C.byref(cfg))
         cfg.num_samples, cfg.num_warmup, cfg.save_warmup, cfg.thin = num_samples, num_warmup, int(save_warmup), thin
@@ -136,6 +138,9 @@ class Model:
         cfg.num_chains, cfg.id, cfg.seed, cfg.fused = num_chains, id, seed, fused
         cfg.precision = {"fp64": 0, "fp32": 1, "bf16": 2}.get(precision, precision)
         cfg.max_saved_params = max_saved_params
+        cfg.flow, cfg.flow_layers, cfg.flow_hidden = int(flow), flow_layers, flow_hidden
+        cfg.flow_train_steps, cfg.flow_batch, cfg.flow_lr = flow_train_steps, flow_batch, flow_lr
+        cfg.flow_warmup, cfg.flow_pilot = flow_warmup, flow_pilot
         init_arr = None
         if isinstance(init, (int, float)):
             cfg.init_radius = float(init)
@@ -189,7 +194,10 @@ class Model:
                   status=status, warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds,
                   leaf_steps=out.leaf_steps, grad_seconds_per_launch=out.grad_seconds_per_launch,
                   grad_launches_timed=out.grad_launches_timed,
-                  grad_chains_per_launch=out.grad_chains_per_launch, column_names=names,
+                  grad_chains_per_launch=out.grad_chains_per_launch, flow_used=bool(out.flow_used),
+                  flow_train_seconds=out.flow_train_seconds, flow_loss=out.flow_loss,
+                  pilot_ess_per_grad_plain=out.pilot_ess_per_grad_plain,
+                  pilot_ess_per_grad_flow=out.pilot_ess_per_grad_flow, column_names=names,
                   dataset_of_chain=ds_of_chain, config=cfg, error=err)
         if rc != 0:
             raise SamplerError(err)
@@ -229,3 +237,72 @@ def data_from_json(path):
 
 
 __all__ = ["Model", "Data", "Fit", "SamplerError", "SAMPLER_COLUMNS", "json", "data_from_json"]
+
+
+class Flow:
+    """The affine-coupling flow of the learned reparameterisation (GPU), for
+    direct use and for its parity tests.  Row arrays are [n][D] float64."""
+
+    def __init__(self, D, n_layers=8, hidden=128, n_glob=0, group_size=1, max_rows=256, seed=1):
+        self.h = lib().nuts_flow_create(D, n_layers, hidden, n_glob, group_size, max_rows, seed)
+        if not self.h:
+            raise SamplerError(last_error())
+        self.D, self.max_rows = D, max_rows
+        self.n_params = lib().nuts_flow_num_params(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().nuts_flow_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SamplerError(last_error())
+
+    def get_params(self):
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self._check(lib().nuts_flow_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(lib().nuts_flow_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def set_standardization(self, loc, scale):
+        self._check(lib().nuts_flow_set_standardization(self.h, _dp(_f64(loc)), _dp(_f64(scale))))
+
+    def _rows(self, a):
+        a = _f64(np.atleast_2d(a))
+        return a, a.shape[0]
+
+    def forward(self, z):
+        z, n = self._rows(z)
+        x, ld = np.zeros_like(z), np.zeros(n)
+        self._check(lib().nuts_flow_forward(self.h, _dp(z), n, _dp(x), _dp(ld)))
+        return x, ld
+
+    def inverse(self, x):
+        x, n = self._rows(x)
+        z, ld = np.zeros_like(x), np.zeros(n)
+        self._check(lib().nuts_flow_inverse(self.h, _dp(x), n, _dp(z), _dp(ld)))
+        return z, ld
+
+    def pullback_grad(self, z, gx):
+        z, n = self._rows(z)
+        gx = _f64(np.atleast_2d(gx))
+        gz = np.zeros_like(z)
+        self._check(lib().nuts_flow_pullback_grad(self.h, _dp(z), _dp(gx), n, _dp(gz)))
+        return gz
+
+    def loss_grad(self, x):
+        x, n = self._rows(x)
+        loss = C.c_double()
+        g = np.zeros(self.n_params, dtype=np.float32)
+        self._check(lib().nuts_flow_loss_grad(self.h, _dp(x), n, C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, g
+
+    def train(self, x, steps, batch=256, lr=1e-3, seed=1):
+        x, n = self._rows(x)
+        loss = C.c_double()
+        self._check(lib().nuts_flow_train(self.h, _dp(x), n, steps, batch, lr, seed, C.byref(loss)))
+        return loss.value

@@ -107,6 +107,21 @@ typedef struct nuts_config {
   int max_saved_params;   /* -1 (default): draws hold the constrained parameters; k >= 0: draws hold
                              the first k unconstrained coordinates only (bounded output for very
                              large batches) */
+  /* Learned reparameterisation (no counterpart in CmdStan): flow = 1 runs plain
+   * adaptive warmup plus flow_pilot draws, trains an affine-coupling flow on the
+   * second half of warmup and the pilot draws, runs flow_warmup adaptive
+   * iterations and flow_pilot draws of NUTS on the pulled-back density, and then
+   * samples num_samples iterations with whichever of the two gave the larger
+   * min bulk-ESS per gradient evaluation over the pilot (plain NUTS is the
+   * fallback).  One dataset; at least 2 dimensions; lockstep kernels. */
+  int flow;               /* 0 (default) off, 1 on */
+  int flow_layers;        /* 8 coupling layers */
+  int flow_hidden;        /* 128: width of the two tanh hidden layers */
+  int flow_train_steps;   /* 1000 Adam steps */
+  int flow_batch;         /* 256 draws per step */
+  double flow_lr;         /* 1e-3 */
+  int flow_warmup;        /* 300 adaptive iterations in the flow's coordinates */
+  int flow_pilot;         /* 100 pilot draws per chain for each of the two samplers */
 } nuts_config;
 
 void nuts_config_default(nuts_config* cfg);
@@ -144,6 +159,12 @@ typedef struct nuts_output {
   int64_t grad_launches_timed;
   double grad_chains_per_launch;  /* lockstep GLM path: mean number of active chains a
                                      gradient launch evaluated (0 elsewhere) */
+  /* flow = 1 only */
+  int flow_used;                  /* 1: the draws come from NUTS on the pulled-back density */
+  double flow_train_seconds;      /* GPU training time */
+  double flow_loss;               /* mean negative log-likelihood of the last training batches */
+  double pilot_ess_per_grad_plain;
+  double pilot_ess_per_grad_flow;
 } nuts_output;
 
 /* Runs warmup then sampling for every chain of every dataset in one batch.
@@ -168,6 +189,35 @@ int nuts_transition_replay(nuts_model* m, int dataset, int num_chains, int metri
  * for n_params parameters; draws[p][chain][draw].  out[p][10] = Mean, MCSE,
  * StdDev, MAD, 5%, 50%, 95%, ESS_bulk, ESS_tail, R_hat.  Host code. */
 int nuts_summary(const double* draws, int n_draws, int n_chains, int n_params, int n_threads, double* out);
+
+/* ------------------------------------------------------ learned flow */
+/* Affine-coupling normalizing flow used by nuts_config.flow = 1, exposed for
+ * its parity tests.  No counterpart in the reference.  x = loc + scale *
+ * f_{L-1}(...f_0(z)); layer l keeps dims a_l and maps b_l by
+ * v_b = u_b exp(s) + t, (s_raw, t) = MLP(u_a) with two tanh layers of width
+ * `hidden`, s = 3 tanh(s_raw / 3).  Masks: n_glob leading global dims, then
+ * groups of group_size dims (group effects conditioned on the globals).
+ * Parameters are FP32; all row buffers are host [n][D] FP64 with n <= max_rows. */
+typedef struct nuts_flow nuts_flow;
+nuts_flow* nuts_flow_create(int D, int n_layers, int hidden, int n_glob, int group_size, int max_rows,
+                            uint64_t seed);
+void nuts_flow_destroy(nuts_flow* f);
+int64_t nuts_flow_num_params(const nuts_flow* f);
+/* layout per layer: W1 [Da][hidden], b1, W2 [hidden][hidden], b2, W3 [hidden][2 Db], b3 */
+int nuts_flow_get_params(const nuts_flow* f, float* out);
+int nuts_flow_set_params(nuts_flow* f, const float* in);
+int nuts_flow_set_standardization(nuts_flow* f, const double* loc, const double* scale);
+/* x = T(z), logdet = log|det dT/dz| */
+int nuts_flow_forward(nuts_flow* f, const double* z, int n, double* x, double* logdet);
+/* z = T^{-1}(x), logdet = log|det dT^{-1}/dx| */
+int nuts_flow_inverse(nuts_flow* f, const double* x, int n, double* z, double* logdet);
+/* gz = d/dz [V(T(z)) - log|det dT/dz|] given gx = grad V at x = T(z) */
+int nuts_flow_pullback_grad(nuts_flow* f, const double* z, const double* gx, int n, double* gz);
+/* mean over rows of 0.5 |T^{-1}(x)|^2 - log|det dT^{-1}/dx| and its parameter gradient */
+int nuts_flow_loss_grad(nuts_flow* f, const double* x, int n, double* loss, float* grad);
+/* Adam on minibatches of `batch` rows drawn from the n rows of x */
+int nuts_flow_train(nuts_flow* f, const double* x, int64_t n, int steps, int batch, double lr, uint64_t seed,
+                    double* final_loss);
 
 const char* nuts_last_error(void);
 const char* nuts_version(void);

@@ -184,6 +184,68 @@ def bench_c3(args):
             "speedup_batched_vs_plain": plain_total / fit.sampling_seconds}
 
 
+def make_c4(seed, N=50000, G=400, K=10):
+    """Centered hierarchical logistic regression: 400 groups of 125 observations,
+    10 covariates N(0,1) held in FP32, beta_g ~ N(mu, tau), tau ~ HalfNormal(1)."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((N, K)).astype(np.float32).astype(np.float64)
+    grp = np.repeat(np.arange(G), N // G)
+    mu = rng.normal(0, 1, K)
+    tau = np.abs(rng.normal(0, 1, K))
+    beta = mu + tau * rng.standard_normal((G, K))
+    eta = np.einsum("nk,nk->n", X, beta[grp])
+    y = (rng.random(N) < 1 / (1 + np.exp(-eta))).astype(np.int64)
+    return dict(N=N, K=K, G=G, X=X, y=y, group=grp + 1)
+
+
+def bench_c4(args):
+    """Learned flow vs plain NUTS on the centered hierarchical logistic model
+    (D = 4020, funnel geometry), 256 chains each."""
+    import cmdstan_amd as ca
+    from cmdstan_amd.summary import summarize
+    data = make_c4(args.seed)
+    m = ca.Model("hier_logistic_cp", data)
+    D = m.dims[0]
+
+    def min_ess(fit):
+        arr = np.stack(fit.draws)  # chains x draws x (7 + D)
+        params = np.concatenate([arr[:, :, 7:7 + D], arr[:, :, :1]], axis=2).transpose(2, 0, 1)
+        st = summarize(params)
+        return float(np.nanmin(st[:, 7])), float(np.nanmax(st[:, 9])), float(arr[:, :, 4].sum())
+
+    out = {}
+    for name, kw in [("plain", {}),
+                     ("flow", dict(flow=True, flow_layers=8, flow_hidden=128, flow_train_steps=args.flow_steps,
+                                   flow_batch=256, flow_lr=1e-3, flow_warmup=args.flow_warmup,
+                                   flow_pilot=args.flow_pilot))]:
+        t0 = time.time()
+        print(f"c4 {name}: sampling", file=sys.stderr, flush=True)
+        fit = m.sample(num_chains=256, num_warmup=args.warmup, num_samples=args.steps, seed=args.seed, refresh=50,
+                       **kw)
+        wall = time.time() - t0
+        ess, rhat, grads = min_ess(fit)
+        r = dict(min_bulk_ess=ess, rhat_max=rhat, sampling_seconds=fit.sampling_seconds,
+                 warmup_seconds=fit.warmup_seconds, wall_seconds=wall,
+                 ess_per_sec_sampling=ess / fit.sampling_seconds,
+                 ess_per_sec_total=ess / (fit.sampling_seconds + fit.warmup_seconds),
+                 ess_per_grad=ess / grads, grad_evals_per_sec=grads / fit.sampling_seconds,
+                 divergent=float(np.stack(fit.draws)[:, :, 5].sum()))
+        if kw:
+            r.update(flow_used=fit.flow_used, flow_train_seconds=fit.flow_train_seconds, flow_loss=fit.flow_loss,
+                     pilot_ess_per_grad_plain=fit.pilot_ess_per_grad_plain,
+                     pilot_ess_per_grad_flow=fit.pilot_ess_per_grad_flow)
+        out[name] = r
+        print(json.dumps({name: r}), file=sys.stderr, flush=True)
+    return {"metric": "min bulk-ESS/sec, learned flow vs plain NUTS (centered hierarchical logistic)",
+            "value": out["flow"]["ess_per_sec_sampling"], "unit": "ESS/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "dtype": "f64 sampler, f32 flow", "data": "synthetic",
+            "config": {"workload": "centered hierarchical logistic N=50000, 400 groups x 10 covariates (D=4020), "
+                                   "256 chains, diag_e; flow 8 coupling layers, MLP 2x128 FP32"},
+            "plain": out["plain"], "flow": out["flow"],
+            "speedup_sampling": out["flow"]["ess_per_sec_sampling"] / out["plain"]["ess_per_sec_sampling"],
+            "speedup_total": out["flow"]["ess_per_sec_total"] / out["plain"]["ess_per_sec_total"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,9 +258,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"],
                     help="GLM contraction precision (sums, H and tree arithmetic stay FP64)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"],
                     help="c2 (default): the headline workload; c1, c3: secondary measurements")
     ap.add_argument("--models", type=int, default=512, help="c3: number of models")
+    ap.add_argument("--flow-steps", type=int, default=1000, help="c4: Adam steps")
+    ap.add_argument("--flow-warmup", type=int, default=300, help="c4: warmup iterations on the flow")
+    ap.add_argument("--flow-pilot", type=int, default=100, help="c4: pilot draws per sampler")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run a device copy of BYTES (counter calibration for profiles)")
     args = ap.parse_args()
@@ -214,6 +279,9 @@ def main():
         return
     if args.config == "c3":
         print(json.dumps(bench_c3(args)))
+        return
+    if args.config == "c4":
+        print(json.dumps(bench_c4(args)))
         return
 
     import cmdstan_amd as ca

@@ -23,6 +23,7 @@ class nuts_config(C.Structure):
         ("fused", C.c_int), ("precision", C.c_int), ("max_saved_params", C.c_int),
         ("flow", C.c_int), ("flow_layers", C.c_int), ("flow_hidden", C.c_int), ("flow_train_steps", C.c_int),
         ("flow_batch", C.c_int), ("flow_lr", C.c_double), ("flow_warmup", C.c_int), ("flow_pilot", C.c_int),
+        ("refresh", C.c_int),
     ]
 
 

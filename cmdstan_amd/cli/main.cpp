@@ -300,6 +300,9 @@ int run(int argc, char** argv) {
   const double radius = std::strtod(init.c_str(), &ie);
   if (init.empty() || *ie) throw std::invalid_argument("init files are not supported by this build; give a radius");
   cfg.init_radius = radius;
+  const long long refresh = to_int(val(R, {"output", "refresh"}), "refresh");
+  if (refresh < 0) throw std::invalid_argument("refresh must be >= 0");
+  cfg.refresh = (int)refresh;
   if (save_metric && !cfg.adapt_engaged)
     throw std::invalid_argument("Cannot save adapted metric if adaptation is disabled.");  // command.hpp:249-259
   if (nuts_config_validate(&cfg) != NUTS_OK) throw std::invalid_argument(nuts_last_error());

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -20,6 +21,7 @@
 #include "json.h"
 #include "launch.h"
 #include "flow.h"
+#include "hier_logistic.h"
 #include "nuts_types.h"
 
 using namespace nutsgpu;
@@ -111,6 +113,7 @@ namespace {
 struct Dataset {
   int D = 0, nout = 0;
   int n_glob = 0, group_size = 1;  // flow coupling masks: leading global dims, then groups
+  int groups = 0;                  // hier_logistic_cp: G (the block itself lives on the device only)
   std::vector<std::string> names, unames;
   std::vector<double> block;
 };
@@ -235,6 +238,7 @@ Dataset make_dataset(const std::string& fam, const nuts_data& d, int& kind) {
       for (int k = 0; k < K; ++k) s.unames.push_back("beta." + std::to_string(g + 1) + "." + std::to_string(k + 1));
     s.n_glob = 2 * K;
     s.group_size = K;
+    s.groups = G;
     s.block = {(double)N, (double)K, (double)G, d.scalar_or("mu_scale", 5.0)};
     s.block.insert(s.block.end(), gstart.begin(), gstart.end());
     for (int n : order) for (int k = 0; k < K; ++k) s.block.push_back(X[(size_t)n * K + k]);
@@ -361,6 +365,20 @@ void validate(const nuts_config& c) {
     if (!c.adapt_engaged) throw std::invalid_argument("flow: adaptation must be engaged");
     if (c.save_warmup) throw std::invalid_argument("flow: save_warmup is not supported");
   }
+}
+
+// iterations completed by the slowest chain (one block)
+__global__ void min_iter_kernel(const ChainState* st, int C, int* out) {
+  __shared__ int m[256];
+  int v = 0x7fffffff;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) v = min(v, st[c].iter);
+  m[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) m[threadIdx.x] = min(m[threadIdx.x], m[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = m[0];
 }
 
 __global__ void resume_kernel(ChainState* st, int C) {
@@ -533,7 +551,9 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
 
   bool fused = cfg.fused == 1;
   const bool glm_ok = M.glm && nds == 1 && !R.flow;
-  if (cfg.fused < 0) fused = !glm_ok;
+  // hierarchical logistic: batched group kernel (lane = chain) over one dataset
+  const bool hl_ok = M.kind == MK_HIER_LOGISTIC && nds == 1;
+  if (cfg.fused < 0) fused = !glm_ok && !hl_ok;
   if (R.flow) {
     if (nds != 1 || R.flow->dim() != D[0] || R.flow->max_rows() < C)
       throw std::invalid_argument("flow: one dataset whose dimension matches the flow, at most max_rows chains");
@@ -546,11 +566,38 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   const void* xlow = nullptr;
   GlmArgs ga;
   GlmPlan gplan;
-  DevBuf<double> fX, fGX, fVX, fLD;
-  if (R.flow) {
+  // large chains: one 256-thread block per chain for the state machine
+  int maxD = 0;
+  for (int c = 0; c < C; ++c) maxD = std::max(maxD, D[c]);
+  const char* bpc = std::getenv("CMDSTAN_AMD_BLOCK_PER_CHAIN");
+  const bool block_per_chain = !fused && (bpc ? std::atoi(bpc) != 0 : maxD >= 512);
+  DevBuf<double> hlwork;
+  HlArgs hla{};
+  if (hl_ok && !fused) {
+    const Dataset& hd = M.ds[R.first_dataset];
+    hlwork.alloc(hl_work_doubles(hd.groups, hd.group_size, C));
     act_idx.alloc(C);
     n_act.alloc(1);
+    hla.block = M.d_data.p + M.ds_off[R.first_dataset];
+    hla.idx = act_idx.p; hla.n_dev = n_act.p; hla.n_rows = C; hla.work = hlwork.p; hla.jac = 1.0;
+    if (R.flow) {  // packed rows x = T(z) in, packed gradient out
+      hla.idx_in = false; hla.idx_out = false;
+    } else {
+      hla.Q = Q.p; hla.ldq = Dp[0]; hla.idx_in = true; hla.idx_out = true;
+      hla.Gout = G.p; hla.ldg = Dp[0]; hla.Vout = V.p;
+    }
+  }
+  const int hl_groups = hl_ok ? M.ds[R.first_dataset].groups : 0;
+  DevBuf<double> fX, fGX, fVX, fLD;
+  if (R.flow) {
+    if (!act_idx.p) {
+      act_idx.alloc(C);
+      n_act.alloc(1);
+    }
     fX.alloc((size_t)C * D[0]); fGX.alloc((size_t)C * D[0]); fVX.alloc(C); fLD.alloc(C);
+    if (hl_ok) {
+      hla.Q = fX.p; hla.ldq = D[0]; hla.Gout = fGX.p; hla.ldg = D[0]; hla.Vout = fVX.p;
+    }
   }
   if (!fused && glm_ok) {
     act_idx.alloc(C);
@@ -590,6 +637,9 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   const char* lim = std::getenv("CMDSTAN_AMD_MAX_SECONDS");
   const double max_seconds = lim ? std::atof(lim) : 3600.0;
   const auto t_begin = std::chrono::steady_clock::now();
+  DevBuf<int> miter(1);
+  int min_it = 0;
+  int next_report = cfg.refresh > 0 ? cfg.refresh : 0;
   auto run_phase = [&]() {
     const int check_every = fused ? 1 : kCheck;
     for (;;) {
@@ -599,7 +649,8 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         if (fused) {
           launch_advance_fused(M.kind, cfg.metric, P, B, md, st);
         } else {
-          launch_advance_external(cfg.metric, P, B, st);
+          if (block_per_chain) launch_advance_external_block(cfg.metric, P, B, st);
+          else launch_advance_external(cfg.metric, P, B, st);
           if (glm_ok) {
             launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
             ga.ev_begin = ev.a[k];
@@ -612,8 +663,14 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
             launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
             HIPC(hipEventRecord(ev.a[k], st));
             R.flow->forward(Q.p, Dp[0], act_idx.p, n_act.p, C, fX.p, fLD.p);
-            launch_model_grad_rows(M.kind, act_idx.p, n_act.p, C, D[0], fX.p, fGX.p, fVX.p, md, st);
+            if (hl_ok) hl_grad_launch(hla, hl_groups, st);
+            else launch_model_grad_rows(M.kind, act_idx.p, n_act.p, C, D[0], fX.p, fGX.p, fVX.p, md, st);
             R.flow->pullback(fGX.p, fVX.p, fLD.p, act_idx.p, n_act.p, C, G.p, Dp[0], V.p);
+            HIPC(hipEventRecord(ev.b[k], st));
+          } else if (hl_ok) {
+            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+            HIPC(hipEventRecord(ev.a[k], st));
+            hl_grad_launch(hla, hl_groups, st);
             HIPC(hipEventRecord(ev.b[k], st));
           } else {
             HIPC(hipEventRecord(ev.a[k], st));
@@ -625,7 +682,22 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
       }
       HIPC(hipGetLastError());
       HIPC(hipMemcpyAsync(&stopped, dstop.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      if (cfg.refresh > 0) {
+        hipLaunchKernelGGL(min_iter_kernel, dim3(1), dim3(256), 0, st, dst.p, C, miter.p);
+        HIPC(hipMemcpyAsync(&min_it, miter.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      }
       HIPC(hipStreamSynchronize(st));
+      if (cfg.refresh > 0) {
+        // generate_transitions' progress line, for the slowest chain
+        const int total = cfg.num_warmup + cfg.num_samples;
+        while (next_report <= min_it && next_report <= total) {
+          std::fprintf(stderr, "Iteration: %*d / %d [%3d%%]  (%s)\n", (int)std::to_string(total).size(), next_report,
+                       total, (int)(100.0 * next_report / std::max(1, total)),
+                       next_report <= cfg.num_warmup ? "Warmup" : "Sampling");
+          std::fflush(stderr);
+          next_report += cfg.refresh;
+        }
+      }
       for (int k = 0; k < (fused ? 0 : check_every); ++k) {
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, ev.a[k], ev.b[k]));
@@ -1013,6 +1085,16 @@ static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n
                                      : glm_grad_mfma_f32(ga, gp, M.low_x(precision, 0), precision, kF32Layout, 0);
       if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
       HIPC(hipDeviceSynchronize());
+    } else if (M.kind == MK_HIER_LOGISTIC) {
+      // the sampler's batched group kernel (lane = point)
+      const Dataset& hd = M.ds[dataset];
+      DevBuf<double> work(hl_work_doubles(hd.groups, hd.group_size, C));
+      HlArgs a{};
+      a.block = M.d_data.p + M.ds_off[dataset];
+      a.Q = Q.p; a.ldq = Dp; a.idx = nullptr; a.idx_in = false; a.idx_out = false; a.n_dev = nullptr; a.n_rows = C;
+      a.Gout = G.p; a.ldg = Dp; a.Vout = V.p; a.work = work.p; a.jac = jacobian ? 1.0 : 0.0;
+      hl_grad_launch(a, hd.groups, 0);
+      HIPC(hipDeviceSynchronize());
     } else {
       launch_model_grad(M.kind, C, dD.p, doff.p, Q.p, G.p, V.p, nullptr, md, 0);
     }
@@ -1048,6 +1130,7 @@ void nuts_config_default(nuts_config* c) {
   c->max_saved_params = -1;
   c->flow = 0; c->flow_layers = 8; c->flow_hidden = 128; c->flow_train_steps = 1000; c->flow_batch = 256;
   c->flow_lr = 1e-3; c->flow_warmup = 300; c->flow_pilot = 100;
+  c->refresh = 0;
 }
 
 int nuts_config_validate(const nuts_config* cfg) {

@@ -27,6 +27,8 @@ struct ModelDataDev {
 // Advance every chain to its next gradient request (lockstep) or through up
 // to P.fused_budget gradients computed in place by the model (fused).
 void launch_advance_external(int metric, const NutsParams& P, const NutsBuffers& B, hipStream_t st);
+// the same with one 256-thread block per chain (large D)
+void launch_advance_external_block(int metric, const NutsParams& P, const NutsBuffers& B, hipStream_t st);
 // Cholesky factor of every chain's dense inverse metric (B.dM -> B.dL); counts failures.
 void launch_dense_factor(const NutsBuffers& B, int* failed, hipStream_t st);
 void launch_advance_fused(int kind, int metric, const NutsParams& P, const NutsBuffers& B, const ModelDataDev& md,

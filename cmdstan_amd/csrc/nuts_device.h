@@ -14,10 +14,10 @@
 // separate kernel (GEMM for GLMs); in the fused mode a model device function is
 // called in place and the chain keeps going for a bounded number of gradients.
 //
-// All lanes of the wave execute the same scalar control flow; vectors are
-// strided over the 64 lanes and reductions are butterfly shuffles, so every
-// lane holds identical scalar state (the staged ChainState in LDS is written
-// with identical values by all lanes).
+// All threads of a chain execute the same scalar control flow; vectors are
+// strided over the chain's threads and reductions give every thread the same
+// value, so every thread holds identical scalar state (the staged ChainState
+// in LDS is written with identical values by all lanes of a wave).
 //
 // Metrics (stan::mcmc::{unit,diag,dense}_e_metric): the "sharp" momentum
 // Minv·p enters the leapfrog and the U-turn checks.  For unit/diag it is formed
@@ -58,14 +58,21 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-template <int METRIC, class GradFn>
+// TPC threads per chain: 64 (one wave) or a multiple of 64 (one block per
+// chain, for large D).  With several waves every wave keeps its own identical
+// copy of the scalar state; vectors are strided over the TPC threads, so an
+// element is always touched by the same thread, and cross-thread reads of
+// vectors (reductions, matvecs, factorisations) go through gsum/gsync, which
+// every wave reaches at the same point because the control flow is identical.
+template <int METRIC, class GradFn, int TPC = NUTS_WAVE>
 struct NutsWave {
+  static_assert(TPC % NUTS_WAVE == 0 && TPC <= 256, "threads per chain");
   static constexpr bool DENSE = METRIC == METRIC_DENSE;
   const NutsParams& P;
   const NutsBuffers& B;
   ChainState& s;
   const GradFn& grad;
-  int c, lane, D, Dp;
+  int c, lane, tid, D, Dp;  // lane in the wave, tid in the chain's thread group
   uint32_t uid;
   double* q;     // Z end position (dense Q row)
   double* g;     // Z end potential gradient (dense G row)
@@ -80,8 +87,9 @@ struct NutsWave {
   double v_new = 0;
   bool have_v = false;
 
-  __device__ NutsWave(const NutsParams& P_, const NutsBuffers& B_, ChainState& s_, const GradFn& g_, int c_, int lane_)
-      : P(P_), B(B_), s(s_), grad(g_), c(c_), lane(lane_) {
+  __device__ NutsWave(const NutsParams& P_, const NutsBuffers& B_, ChainState& s_, const GradFn& g_, int c_, int lane_,
+                      int tid_)
+      : P(P_), B(B_), s(s_), grad(g_), c(c_), lane(lane_), tid(tid_) {
     D = B.D[c];
     Dp = B.Dp[c];
     uid = B.uid[c];
@@ -96,6 +104,57 @@ struct NutsWave {
     } else {
       dM = dL = dW = nullptr;
     }
+  }
+
+  // group reduction: every thread gets the same sum (fixed order)
+  static __device__ double gsum(double v) {
+    v = wave_sum(v);
+    if constexpr (TPC == NUTS_WAVE) {
+      return v;
+    } else {
+      __shared__ double red[TPC / NUTS_WAVE];
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+      __syncthreads();
+      double r = 0.0;
+#pragma unroll
+      for (int w = 0; w < TPC / NUTS_WAVE; ++w) r += red[w];
+      return r;
+    }
+  }
+  // global vector writes of the group visible to its other threads
+  static __device__ void gsync() {
+    if constexpr (TPC == NUTS_WAVE) {
+      wave_sync();
+    } else {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+  }
+  static __device__ int gand(int v) {
+    v = wave_and(v);
+    if constexpr (TPC == NUTS_WAVE) {
+      return v;
+    } else {
+      __shared__ int redi[TPC / NUTS_WAVE];
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) redi[threadIdx.x >> 6] = v;
+      __syncthreads();
+      int r = 1;
+#pragma unroll
+      for (int w = 0; w < TPC / NUTS_WAVE; ++w) r &= redi[w];
+      return r;
+    }
+  }
+  // Reconverges the wave after a lane-strided loop: the scalar state updates
+  // that follow run with the full wave, never duplicated into the divergent
+  // exits of the loop (where a read-modify-write of the shared state would
+  // be applied once per lane group).
+  static __device__ void converge() { __builtin_amdgcn_wave_barrier(); }
+  static __device__ void lsync() {
+    if constexpr (TPC == NUTS_WAVE) lds_sync();
+    else __syncthreads();
   }
 
   __device__ double* vecp(int id) const { return blk + (int64_t)id * Dp; }
@@ -116,9 +175,9 @@ struct NutsWave {
   }
   // out = Minv x (dense), summing j = 0..D-1 in order like Eigen's row product
   __device__ void matvec(const double* x, double* out) const {
-    wave_sync();
+    gsync();
     constexpr int RB = 8;
-    for (int i0 = 0; i0 < D; i0 += NUTS_WAVE * RB) {
+    for (int i0 = 0; i0 < D; i0 += TPC * RB) {
       double acc[RB];
 #pragma unroll
       for (int r = 0; r < RB; ++r) acc[r] = 0.0;
@@ -127,23 +186,23 @@ struct NutsWave {
         const double* col = dM + (int64_t)j * D;  // column j = row j of Minv
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
-          const int i = i0 + lane + NUTS_WAVE * r;
+          const int i = i0 + tid + TPC * r;
           if (i < D) acc[r] += col[i] * xj;
         }
       }
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
-        const int i = i0 + lane + NUTS_WAVE * r;
+        const int i = i0 + tid + TPC * r;
         if (i < D) out[i] = acc[r];
       }
     }
-    wave_sync();
+    gsync();
   }
   // T = 1/2 p·Minv p; for dense ps must hold Minv p
   __device__ double kinetic(const double* p, const double* ps) const {
     double a = 0;
-    for (int i = lane; i < D; i += NUTS_WAVE) a += p[i] * sh(p, ps, i);
-    return 0.5 * wave_sum(a);
+    for (int i = tid; i < D; i += TPC) a += p[i] * sh(p, ps, i); converge();
+    return 0.5 * gsum(a);
   }
 
   // ------------------------------------------------------------- draws
@@ -164,10 +223,10 @@ struct NutsWave {
   // {unit,diag,dense}_e_metric::sample_p; for dense also sets ps = Minv p
   __device__ void sample_p(double* p, double* ps) {
     if (!DENSE) {
-      for (int i = lane; i < D; i += NUTS_WAVE) {
+      for (int i = tid; i < D; i += TPC) {
         const double z = normal_draw(i);
         p[i] = METRIC == METRIC_UNIT ? z : z / sqrt(minv[i]);
-      }
+      } converge();
       advance_normals();
       return;
     }
@@ -175,31 +234,31 @@ struct NutsWave {
     // wave; column i of L below the diagonal is contiguous in the
     // column-major factor
     __shared__ double sp[4][NUTS_WAVE * DENSE_RMAX];
-    double* pw = sp[threadIdx.x >> 6];
-    for (int i = lane; i < D; i += NUTS_WAVE) pw[i] = normal_draw(i);
+    double* pw = sp[TPC == NUTS_WAVE ? (threadIdx.x >> 6) : 0];
+    for (int i = tid; i < D; i += TPC) pw[i] = normal_draw(i); converge();
     advance_normals();
-    lds_sync();
+    lsync();
     for (int i = D - 1; i >= 0; --i) {
       const double* Li = dL + (int64_t)i * D;
       double a = 0;
-      for (int j = i + 1 + lane; j < D; j += NUTS_WAVE) a += Li[j] * pw[j];
-      a = wave_sum(a);
+      for (int j = i + 1 + tid; j < D; j += TPC) a += Li[j] * pw[j]; converge();
+      a = gsum(a);
       const double pi = (pw[i] - a) / Li[i];
-      lds_sync();  // every lane has read z_i before it is overwritten
-      if (lane == 0) pw[i] = pi;
-      lds_sync();
+      lsync();  // every lane has read z_i before it is overwritten
+      if (tid == 0) pw[i] = pi; converge();
+      lsync();
     }
-    for (int i = lane; i < D; i += NUTS_WAVE) p[i] = pw[i];
-    wave_sync();
+    for (int i = tid; i < D; i += TPC) p[i] = pw[i]; converge();
+    gsync();
     matvec(p, ps);
   }
 
   // ------------------------------------------------------------- vectors
   __device__ void copy(double* dst, const double* src) const {
-    for (int i = lane; i < D; i += NUTS_WAVE) dst[i] = src[i];
+    for (int i = tid; i < D; i += TPC) dst[i] = src[i]; converge();
   }
   __device__ void swap(double* a, double* b) const {
-    for (int i = lane; i < D; i += NUTS_WAVE) { double t = a[i]; a[i] = b[i]; b[i] = t; }
+    for (int i = tid; i < D; i += TPC) { double t = a[i]; a[i] = b[i]; b[i] = t; } converge();
   }
   // expl_leapfrog: begin_update_p(eps/2), update_q(eps)
   __device__ void leap_begin(double e) const {
@@ -207,22 +266,22 @@ struct NutsWave {
     const double h = 0.5 * e;
     if (DENSE) {
       double* t = vecp(PV_TMP);
-      for (int i = lane; i < D; i += NUTS_WAVE) p[i] = p[i] - h * g[i];
+      for (int i = tid; i < D; i += TPC) p[i] = p[i] - h * g[i]; converge();
       matvec(p, t);
-      for (int i = lane; i < D; i += NUTS_WAVE) q[i] = q[i] + e * t[i];
+      for (int i = tid; i < D; i += TPC) q[i] = q[i] + e * t[i]; converge();
       return;
     }
-    for (int i = lane; i < D; i += NUTS_WAVE) {
+    for (int i = tid; i < D; i += TPC) {
       const double pi = p[i] - h * g[i];
       p[i] = pi;
       q[i] = q[i] + e * (mi(i) * pi);
-    }
+    } converge();
   }
   // end_update_p(eps/2); dense: also the leaf's sharp momentum
   __device__ void leap_end(double e) const {
     double* p = vecp(PV_P);
     const double h = 0.5 * e;
-    for (int i = lane; i < D; i += NUTS_WAVE) p[i] = p[i] - h * g[i];
+    for (int i = tid; i < D; i += TPC) p[i] = p[i] - h * g[i]; converge();
     if (DENSE) matvec(p, vecp(PV_PS));
   }
 
@@ -253,7 +312,7 @@ struct NutsWave {
   }
   __device__ void stop(int phase) {
     s.phase = phase;
-    if (lane == 0) atomicAdd(B.n_stopped, 1);
+    if (tid == 0) atomicAdd(B.n_stopped, 1); converge();
   }
   __device__ void fail(int code) {
     s.failure = code;
@@ -263,20 +322,20 @@ struct NutsWave {
   // services::util::initialize
   __device__ bool init_try() {
     const int t = s.init_tries;
-    for (int i = lane; i < D; i += NUTS_WAVE) {
+    for (int i = tid; i < D; i += TPC) {
       double v;
       if (B.user_init) v = B.user_init[B.off[c] + i];
       else if (P.init_radius == 0) v = 0.0;
       else v = P.init_radius * (2.0 * stream_uniform(P.seed, uid, STREAM_INIT, (uint64_t)t * (uint64_t)D + i) - 1.0);
       q[i] = v;
-    }
+    } converge();
     return request(PH_INIT_CHECK);
   }
   __device__ bool init_check() {
     const double V = pending_V();
     int ok = 1;
-    for (int i = lane; i < D; i += NUTS_WAVE) ok &= isfinite(g[i]) ? 1 : 0;
-    ok = wave_and(ok) && isfinite(V);
+    for (int i = tid; i < D; i += TPC) ok &= isfinite(g[i]) ? 1 : 0; converge();
+    ok = gand(ok) && isfinite(V);
     if (ok) {
       const int k = s.sample_slot;
       copy(Sq(k), q);
@@ -359,14 +418,14 @@ struct NutsWave {
     sample_p(p, ps);
     double* oq = vecp(PV_OQ); double* op = vecp(PV_OP); double* og = vecp(PV_OG);
     double* pfb = vecp(PV_PFB); double* pbf = vecp(PV_PBF); double* rho = vecp(PV_RHO);
-    for (int i = lane; i < D; i += NUTS_WAVE) {
+    for (int i = tid; i < D; i += TPC) {
       const double pi = p[i];
       oq[i] = q[i]; op[i] = pi; og[i] = g[i];
       pfb[i] = pi; pbf[i] = pi; rho[i] = pi;
-    }
+    } converge();
     if (DENSE) {
       double* ops = vecp(PV_OPS); double* psfb = vecp(PV_PSFB); double* psbf = vecp(PV_PSBF);
-      for (int i = lane; i < D; i += NUTS_WAVE) { const double v = ps[i]; ops[i] = v; psfb[i] = v; psbf[i] = v; }
+      for (int i = tid; i < D; i += TPC) { const double v = ps[i]; ops[i] = v; psfb[i] = v; psbf[i] = v; } converge();
     }
     s.Vo = s.Vz;
     s.z_is_fwd = 1;
@@ -445,7 +504,7 @@ struct NutsWave {
       const double* les = DENSE ? LSend(k) : nullptr;
       const double* sbs = DENSE ? (k == 0 ? ps : LSbeg(k - 1)) : nullptr;
       double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
-      for (int i = lane; i < D; i += NUTS_WAVE) {
+      for (int i = tid; i < D; i += TPC) {
         const double rs = lr[i] + srho[i];
         const double r2 = lr[i] + sb[i];
         const double r3 = srho[i] + le[i];
@@ -454,9 +513,9 @@ struct NutsWave {
         a2 += slb * r2; b2 += sh(sb, sbs, i) * r2;
         a3 += sh(le, les, i) * r3; b3 += sp * r3;
         srho[i] = rs;
-      }
-      a1 = wave_sum(a1); b1 = wave_sum(b1); a2 = wave_sum(a2);
-      b2 = wave_sum(b2); a3 = wave_sum(a3); b3 = wave_sum(b3);
+      } converge();
+      a1 = gsum(a1); b1 = gsum(b1); a2 = gsum(a2);
+      b2 = gsum(b2); a3 = gsum(a3); b3 = gsum(b3);
       const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
       s.S_logw = lsw_sub;
       if (!ok) { free_slot(s.s_prop); trans_end(); return false; }
@@ -508,7 +567,7 @@ struct NutsWave {
     const double* bbs = fwd ? ops : ps;
     const double* ffs = fwd ? ps : ops;
     double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
-    for (int i = lane; i < D; i += NUTS_WAVE) {
+    for (int i = tid; i < D; i += TPC) {
       const double old = rho[i], nw = srho[i];
       const double rn = fwd ? old + nw : nw + old;  // rho_bck + rho_fwd
       const double rb = fwd ? old : nw, rf = fwd ? nw : old;
@@ -519,9 +578,9 @@ struct NutsWave {
       a2 += sbb * r2; b2 += sh(pfb, psfb, i) * r2;
       a3 += sh(pbf, psbf, i) * r3; b3 += sff * r3;
       rho[i] = rn;
-    }
-    a1 = wave_sum(a1); b1 = wave_sum(b1); a2 = wave_sum(a2);
-    b2 = wave_sum(b2); a3 = wave_sum(a3); b3 = wave_sum(b3);
+    } converge();
+    a1 = gsum(a1); b1 = gsum(b1); a2 = gsum(a2);
+    b2 = gsum(b2); a3 = gsum(a3); b3 = gsum(b3);
     const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
     if (!ok || s.depth >= P.max_depth) trans_end();
     else s.phase = PH_DOUBLING_BEGIN;
@@ -551,26 +610,26 @@ struct NutsWave {
       s.welford_n++;
       const double n = (double)s.welford_n;
       if (!DENSE) {
-        for (int i = lane; i < D; i += NUTS_WAVE) {
+        for (int i = tid; i < D; i += TPC) {
           const double delta = qs[i] - wm[i];
           const double m = wm[i] + delta / n;
           wm[i] = m;
           wm2[i] = wm2[i] + (qs[i] - m) * delta;
-        }
+        } converge();
       } else {
         // welford_covar_estimator: delta = q - m; m += delta/n; m2 += (q - m) deltaᵀ
         double* delta = vecp(PV_TMP);
-        for (int i = lane; i < D; i += NUTS_WAVE) {
+        for (int i = tid; i < D; i += TPC) {
           const double dl = qs[i] - wm[i];
           delta[i] = dl;
           wm[i] = wm[i] + dl / n;
           wm2[i] = qs[i] - wm[i];  // (q - m_new), kept in the diag accumulator slot
-        }
-        wave_sync();
+        } converge();
+        gsync();
         for (int j = 0; j < D; ++j) {
           const double dj = delta[j];
           double* col = dW + (int64_t)j * D;
-          for (int i = lane; i < D; i += NUTS_WAVE) col[i] = col[i] + wm2[i] * dj;
+          for (int i = tid; i < D; i += TPC) col[i] = col[i] + wm2[i] * dj; converge();
         }
       }
     }
@@ -579,33 +638,33 @@ struct NutsWave {
       const double n = (double)s.welford_n;
       int ok = 1;
       if (!DENSE) {
-        for (int i = lane; i < D; i += NUTS_WAVE) {
+        for (int i = tid; i < D; i += TPC) {
           const double var = s.welford_n > 1 ? wm2[i] / (n - 1.0) : minv[i];
           const double v = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0));
           minv[i] = v;
           ok &= isfinite(v) ? 1 : 0;
           wm[i] = 0; wm2[i] = 0;
-        }
+        } converge();
       } else {
         for (int j = 0; j < D; ++j) {
           double* Mj = dM + (int64_t)j * D;
           double* Wj = dW + (int64_t)j * D;
-          for (int i = lane; i < D; i += NUTS_WAVE) {
+          for (int i = tid; i < D; i += TPC) {
             const double cov = s.welford_n > 1 ? Wj[i] / (n - 1.0) : Mj[i];
             const double v = (n / (n + 5.0)) * cov + (i == j ? 1e-3 * (5.0 / (n + 5.0)) : 0.0);
             Mj[i] = v;
             ok &= isfinite(v) ? 1 : 0;
             Wj[i] = 0;
-          }
+          } converge();
         }
-        for (int i = lane; i < D; i += NUTS_WAVE) { wm[i] = 0; wm2[i] = 0; }
+        for (int i = tid; i < D; i += TPC) { wm[i] = 0; wm2[i] = 0; } converge();
       }
       s.welford_n = 0;
       s.w_counter++;
-      if (!wave_and(ok)) { fail(FAIL_METRIC_OVERFLOW); return false; }
+      if (!gand(ok)) { fail(FAIL_METRIC_OVERFLOW); return false; }
       if (DENSE) {
-        wave_sync();  // the factorisation reads elements other lanes just wrote
-        if (!dense_factor(dM, dL, D, lane)) {
+        gsync();  // the factorisation reads elements other lanes just wrote
+        if (!dense_factor(dM, dL, D, tid)) {
           fail(FAIL_METRIC_NOT_PD);
           return false;
         }
@@ -626,7 +685,7 @@ struct NutsWave {
     if (save) {
       const int nsave = P.save_params < 0 ? D : (P.save_params < D ? P.save_params : D);
       double* row = B.draws + B.draw_off[c] + (int64_t)s.draw_idx * (7 + nsave);
-      if (lane == 0) {
+      if (tid == 0) {
         row[0] = -s.slot_V[k];
         row[1] = accept;
         row[2] = s.eps;
@@ -634,9 +693,9 @@ struct NutsWave {
         row[4] = s.n_leapfrog;
         row[5] = s.divergent;
         row[6] = s.slot_H[k];
-      }
+      } converge();
       const double* sq = Sq(k);
-      for (int i = lane; i < nsave; i += NUTS_WAVE) row[7 + i] = sq[i];
+      for (int i = tid; i < nsave; i += TPC) row[7 + i] = sq[i]; converge();
       s.draw_idx++;
     }
     if (s.adapt_flag) {
@@ -688,13 +747,13 @@ struct NutsWave {
       if (s.phase >= PH_PAUSED) return;
       const bool req = dispatch();
       if (!req) continue;
-      if (!GradFn::fused) { if (lane == 0) B.need[c] = 1; return; }
-      wave_sync();  // the model reads q written lane-strided
-      const double V = grad(c, q, g, D, lane);
+      if (!GradFn::fused) { if (tid == 0) B.need[c] = 1; converge(); return; }
+      gsync();  // the model reads q written lane-strided
+      const double V = grad(c, q, g, D, lane);  // fused kernels run one wave per chain
       v_new = V;
       have_v = true;
-      if (lane == 0) B.V[c] = V;  // for the next launch only
-      wave_sync();  // models may write g elements that other lanes read
+      if (tid == 0) B.V[c] = V; converge();  // for the next launch only
+      gsync();  // models may write g elements that other lanes read
       if (--budget == 0) return;
     }
   }
@@ -702,24 +761,24 @@ struct NutsWave {
   // Lower Cholesky factor of the column-major SPD matrix A into L (column-major),
   // left-looking, one column per step with the wave splitting the rows.
   // Returns false if a pivot is not positive.
-  static __device__ bool dense_factor(const double* A, double* L, int D, int lane) {
+  static __device__ bool dense_factor(const double* A, double* L, int D, int tid) {
     for (int j = 0; j < D; ++j) {
       // pivot: A[j][j] - sum_k L[j][k]^2 (row j of L = element j of columns k < j)
       double sq = 0;
-      for (int k = lane; k < j; k += NUTS_WAVE) { const double v = L[(int64_t)k * D + j]; sq += v * v; }
-      sq = wave_sum(sq);
+      for (int k = tid; k < j; k += TPC) { const double v = L[(int64_t)k * D + j]; sq += v * v; } converge();
+      sq = gsum(sq);
       const double piv = A[(int64_t)j * D + j] - sq;
       if (!(piv > 0)) return false;
       const double d = sqrt(piv);
       double* Lj = L + (int64_t)j * D;
-      for (int i = j + 1 + lane; i < D; i += NUTS_WAVE) {
+      for (int i = j + 1 + tid; i < D; i += TPC) {
         double v = A[(int64_t)j * D + i];  // A[i][j] (lower triangle)
         for (int k = 0; k < j; ++k) v -= L[(int64_t)k * D + i] * L[(int64_t)k * D + j];
         Lj[i] = v / d;
-      }
-      for (int i = lane; i < j; i += NUTS_WAVE) Lj[i] = 0.0;
-      if (lane == 0) Lj[j] = d;
-      wave_sync();
+      } converge();
+      for (int i = tid; i < j; i += TPC) Lj[i] = 0.0; converge();
+      if (tid == 0) Lj[j] = d; converge();
+      gsync();
     }
     return true;
   }
@@ -735,13 +794,16 @@ __global__ void __launch_bounds__(256) dense_factor_kernel(NutsBuffers B, int* f
   if (!ok && lane == 0) atomicAdd(failed, 1);
 }
 
-// Stage ChainState in LDS, run, write back.  One wave per chain; 4 waves per block.
-template <int METRIC, class GradFn>
+// Stage ChainState in LDS, run, write back.  TPC == 64: one wave per chain, 4
+// chains per block; TPC == 256: one chain per block, each wave with its own
+// identical copy of the state, wave 0 writes it back.
+template <int METRIC, class GradFn, int TPC = NUTS_WAVE>
 __device__ __forceinline__ void nuts_advance_body(const NutsParams& P, const NutsBuffers& B, const GradFn& grad) {
   __shared__ ChainState sst[4];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + w;
+  const int c = TPC == NUTS_WAVE ? blockIdx.x * 4 + w : blockIdx.x;
+  const int tid = TPC == NUTS_WAVE ? lane : (int)threadIdx.x;
   if (c >= B.C) return;
   ChainState& s = sst[w];
   {
@@ -751,13 +813,15 @@ __device__ __forceinline__ void nuts_advance_body(const NutsParams& P, const Nut
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  // every wave has its copy of the state before wave 0 may write the new one back
+  if (TPC > NUTS_WAVE) __syncthreads();
   if (s.phase < PH_PAUSED) {
-    NutsWave<METRIC, GradFn> wv(P, B, s, grad, c, lane);
+    NutsWave<METRIC, GradFn, TPC> wv(P, B, s, grad, c, lane, tid);
     wv.run();
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  {
+  if (TPC == NUTS_WAVE || w == 0) {
     uint32_t* dst = reinterpret_cast<uint32_t*>(B.st + c);
     const uint32_t* src = reinterpret_cast<const uint32_t*>(&s);
     for (int i = lane; i < (int)(sizeof(ChainState) / 4); i += 64) dst[i] = src[i];

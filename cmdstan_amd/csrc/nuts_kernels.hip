@@ -13,6 +13,12 @@ __global__ void __launch_bounds__(256) advance_external_kernel(NutsParams P, Nut
   nuts_advance_body<METRIC>(P, B, ExternalGrad{});
 }
 
+// one 256-thread block per chain (large D)
+template <int METRIC>
+__global__ void __launch_bounds__(256) advance_external_block_kernel(NutsParams P, NutsBuffers B) {
+  nuts_advance_body<METRIC, ExternalGrad, 256>(P, B, ExternalGrad{});
+}
+
 template <int METRIC, class F>
 __global__ void __launch_bounds__(256) advance_fused_kernel(NutsParams P, NutsBuffers B, F f) {
   nuts_advance_body<METRIC>(P, B, f);
@@ -64,6 +70,14 @@ void launch_advance_external(int metric, const NutsParams& P, const NutsBuffers&
   if (metric == METRIC_UNIT) hipLaunchKernelGGL(advance_external_kernel<METRIC_UNIT>, grid, blk, 0, st, P, B);
   else if (metric == METRIC_DENSE) hipLaunchKernelGGL(advance_external_kernel<METRIC_DENSE>, grid, blk, 0, st, P, B);
   else hipLaunchKernelGGL(advance_external_kernel<METRIC_DIAG>, grid, blk, 0, st, P, B);
+}
+
+void launch_advance_external_block(int metric, const NutsParams& P, const NutsBuffers& B, hipStream_t st) {
+  const dim3 grid(B.C), blk(256);
+  if (metric == METRIC_UNIT) hipLaunchKernelGGL(advance_external_block_kernel<METRIC_UNIT>, grid, blk, 0, st, P, B);
+  else if (metric == METRIC_DENSE)
+    hipLaunchKernelGGL(advance_external_block_kernel<METRIC_DENSE>, grid, blk, 0, st, P, B);
+  else hipLaunchKernelGGL(advance_external_block_kernel<METRIC_DIAG>, grid, blk, 0, st, P, B);
 }
 
 void launch_dense_factor(const NutsBuffers& B, int* failed, hipStream_t st) {

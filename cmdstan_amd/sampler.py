@@ -127,7 +127,7 @@ class Model:
                window=25, max_depth=10, metric="diag_e", stepsize=1.0, stepsize_jitter=0.0, id=1, seed=0,
                init=2.0, inv_metric=None, fused=-1, precision=0, max_saved_params=-1, flow=False, flow_layers=8,
                flow_hidden=128, flow_train_steps=1000, flow_batch=256, flow_lr=1e-3, flow_warmup=300,
-               flow_pilot=100):
+               flow_pilot=100, refresh=0):
         cfg = nuts_config()
         lib().nuts_config_default(C.byref(cfg))
         cfg.num_samples, cfg.num_warmup, cfg.save_warmup, cfg.thin = num_samples, num_warmup, int(save_warmup), thin
@@ -141,6 +141,7 @@ class Model:
         cfg.flow, cfg.flow_layers, cfg.flow_hidden = int(flow), flow_layers, flow_hidden
         cfg.flow_train_steps, cfg.flow_batch, cfg.flow_lr = flow_train_steps, flow_batch, flow_lr
         cfg.flow_warmup, cfg.flow_pilot = flow_warmup, flow_pilot
+        cfg.refresh = refresh
         init_arr = None
         if isinstance(init, (int, float)):
             cfg.init_radius = float(init)

@@ -122,6 +122,8 @@ typedef struct nuts_config {
   double flow_lr;         /* 1e-3 */
   int flow_warmup;        /* 300 adaptive iterations in the flow's coordinates */
   int flow_pilot;         /* 100 pilot draws per chain for each of the two samplers */
+  int refresh;            /* 0 (default): silent; k > 0: progress line on stderr each time the
+                             slowest chain passes a multiple of k iterations (CmdStan's refresh=) */
 } nuts_config;
 
 void nuts_config_default(nuts_config* cfg);

@@ -313,3 +313,56 @@ def test_dense_deterministic_and_valid_over_seeds():
         assert np.all(a.status == 0)
         for x, y in zip(a.draws, b.draws):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("jacobian", [True, False])
+def test_hier_logistic_batched_gradient_large_groups(jacobian):
+    """The batched group kernel (lane = point, groups staged through LDS in
+    passes of 128 observations) on uneven groups of up to ~900 observations and
+    K = 10, against the oracle at 1e-12 of the conditioned scale."""
+    from tests.model_cases import hier_logistic
+    data = hier_logistic(seed=4, N=5000, K=10, G=9)
+    gm, om = ca.Model("hier_logistic_cp", data), O.Model("hier_logistic_cp", data)
+    rng = np.random.default_rng(8)
+    q = rng.standard_normal((70, gm.dim)) * 0.6
+    lp, g = gm.log_prob_grad(q, jacobian=jacobian)
+    for i in range(len(q)):
+        lp_ref, g_ref = om.log_prob_grad(q[i], jacobian)
+        assert abs(lp[i] - lp_ref) <= 1e-11 * max(1.0, abs(lp_ref))
+        assert np.all(np.abs(g[i] - g_ref) <= 1e-12 * np.maximum(1.0, np.abs(g_ref)) * 64)
+
+
+def test_hier_logistic_lockstep_matches_fused_posterior():
+    """Batched lockstep kernels and the fused per-chain kernel sample the same
+    posterior (means within 4 combined MCSE)."""
+    from tests.model_cases import hier_logistic
+    data = hier_logistic(seed=2, N=600, K=3, G=5)
+    m = ca.Model("hier_logistic_cp", data)
+    a = m.sample(num_chains=32, num_warmup=400, num_samples=400, seed=5, fused=0)
+    b = m.sample(num_chains=32, num_warmup=400, num_samples=400, seed=6, fused=1)
+    assert np.all(a.status == 0) and np.all(b.status == 0)
+    for name in ["mu.1", "tau.2", "beta.3.1"]:
+        sa, sb = E.summary(a.param(name)), E.summary(b.param(name))
+        assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"]), (name, sa, sb)
+
+
+@pytest.mark.parametrize("metric", ["diag_e", "dense_e"])
+def test_block_per_chain_state_machine_matches_wave_mode(metric, monkeypatch):
+    """The 256-thread-per-chain state machine (used for large D) against the
+    one-wave-per-chain one on the same seeds: a short run agrees to rounding,
+    a long one has the same posterior."""
+    from tests.model_cases import gauss
+    m = ca.Model("gauss", gauss(seed=3, D=40))
+    kw = dict(num_chains=16, seed=21, metric=metric, fused=0)
+    monkeypatch.setenv("CMDSTAN_AMD_BLOCK_PER_CHAIN", "0")
+    a = m.sample(num_warmup=20, num_samples=5, **kw)
+    la = m.sample(num_warmup=300, num_samples=300, **kw)
+    monkeypatch.setenv("CMDSTAN_AMD_BLOCK_PER_CHAIN", "1")
+    b = m.sample(num_warmup=20, num_samples=5, **kw)
+    lb = m.sample(num_warmup=300, num_samples=300, **dict(kw, seed=22))
+    assert np.all(a.status == 0) and np.all(b.status == 0) and np.all(lb.status == 0)
+    for x, y in zip(a.draws, b.draws):
+        assert np.allclose(x, y, rtol=1e-8, atol=1e-8)
+    for i in (1, 7, 40):
+        sa, sb = E.summary(la.param(f"x.{i}")), E.summary(lb.param(f"x.{i}"))
+        assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"])

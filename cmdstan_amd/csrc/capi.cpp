@@ -640,45 +640,75 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   DevBuf<int> miter(1);
   int min_it = 0;
   int next_report = cfg.refresh > 0 ? cfg.refresh : 0;
+  // One check interval of lockstep work: kCheck x (advance, gradient).  With
+  // events, each gradient's contraction (or whole gradient) is bracketed.
+  auto enqueue_bundle = [&](bool with_events) {
+    for (int k = 0; k < kCheck; ++k) {
+      if (block_per_chain) launch_advance_external_block(cfg.metric, P, B, st);
+      else launch_advance_external(cfg.metric, P, B, st);
+      if (glm_ok) {
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+        ga.ev_begin = with_events ? ev.a[k] : nullptr;
+        ga.ev_end = with_events ? ev.b[k] : nullptr;
+        const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
+                                           : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
+        if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+      } else if (R.flow) {
+        // pulled-back gradient: x = T(z) for the active chains, model at x, VJP to z
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+        if (with_events) HIPC(hipEventRecord(ev.a[k], st));
+        R.flow->forward(Q.p, Dp[0], act_idx.p, n_act.p, C, fX.p, fLD.p);
+        if (hl_ok) hl_grad_launch(hla, hl_groups, st);
+        else launch_model_grad_rows(M.kind, act_idx.p, n_act.p, C, D[0], fX.p, fGX.p, fVX.p, md, st);
+        R.flow->pullback(fGX.p, fVX.p, fLD.p, act_idx.p, n_act.p, C, G.p, Dp[0], V.p);
+        if (with_events) HIPC(hipEventRecord(ev.b[k], st));
+      } else if (hl_ok) {
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+        if (with_events) HIPC(hipEventRecord(ev.a[k], st));
+        hl_grad_launch(hla, hl_groups, st);
+        if (with_events) HIPC(hipEventRecord(ev.b[k], st));
+      } else {
+        if (with_events) HIPC(hipEventRecord(ev.a[k], st));
+        launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
+        if (with_events) HIPC(hipEventRecord(ev.b[k], st));
+      }
+    }
+  };
+  // Lockstep launch sequences are replayed from a captured HIP graph (one per
+  // phase: the phase's parameters are baked in).  Default: on for the
+  // launch-bound paths (flow, per-chain models), off for the GLM contraction,
+  // whose kernel is timed with events.
+  const char* genv = std::getenv("CMDSTAN_AMD_GRAPH");
+  const bool use_graph = !fused && (genv ? std::atoi(genv) != 0 : !glm_ok);
+  struct GraphHold {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    void reset() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+      x = nullptr; g = nullptr;
+    }
+    ~GraphHold() { reset(); }
+  } graph;
   auto run_phase = [&]() {
-    const int check_every = fused ? 1 : kCheck;
+    graph.reset();
     for (;;) {
       if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() > max_seconds)
         throw std::runtime_error("sampler did not finish within CMDSTAN_AMD_MAX_SECONDS");
-      for (int k = 0; k < check_every; ++k) {
-        if (fused) {
-          launch_advance_fused(M.kind, cfg.metric, P, B, md, st);
-        } else {
-          if (block_per_chain) launch_advance_external_block(cfg.metric, P, B, st);
-          else launch_advance_external(cfg.metric, P, B, st);
-          if (glm_ok) {
-            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
-            ga.ev_begin = ev.a[k];
-            ga.ev_end = ev.b[k];
-            const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
-                                               : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
-            if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
-          } else if (R.flow) {
-            // pulled-back gradient: x = T(z) for the active chains, model at x, VJP to z
-            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
-            HIPC(hipEventRecord(ev.a[k], st));
-            R.flow->forward(Q.p, Dp[0], act_idx.p, n_act.p, C, fX.p, fLD.p);
-            if (hl_ok) hl_grad_launch(hla, hl_groups, st);
-            else launch_model_grad_rows(M.kind, act_idx.p, n_act.p, C, D[0], fX.p, fGX.p, fVX.p, md, st);
-            R.flow->pullback(fGX.p, fVX.p, fLD.p, act_idx.p, n_act.p, C, G.p, Dp[0], V.p);
-            HIPC(hipEventRecord(ev.b[k], st));
-          } else if (hl_ok) {
-            launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
-            HIPC(hipEventRecord(ev.a[k], st));
-            hl_grad_launch(hla, hl_groups, st);
-            HIPC(hipEventRecord(ev.b[k], st));
-          } else {
-            HIPC(hipEventRecord(ev.a[k], st));
-            launch_model_grad(M.kind, C, dD.p, doffb.p, Q.p, G.p, V.p, dneed.p, md, st);
-            HIPC(hipEventRecord(ev.b[k], st));
-          }
-          res.leaf_steps++;
+      if (fused) {
+        launch_advance_fused(M.kind, cfg.metric, P, B, md, st);
+      } else if (use_graph) {
+        if (!graph.x) {
+          HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          enqueue_bundle(false);
+          HIPC(hipStreamEndCapture(st, &graph.g));
+          HIPC(hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0));
         }
+        HIPC(hipGraphLaunch(graph.x, st));
+        res.leaf_steps += kCheck;
+      } else {
+        enqueue_bundle(true);
+        res.leaf_steps += kCheck;
       }
       HIPC(hipGetLastError());
       HIPC(hipMemcpyAsync(&stopped, dstop.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -698,11 +728,13 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
           next_report += cfg.refresh;
         }
       }
-      for (int k = 0; k < (fused ? 0 : check_every); ++k) {
-        float ms = 0;
-        HIPC(hipEventElapsedTime(&ms, ev.a[k], ev.b[k]));
-        res.grad_s += ms * 1e-3;
-        res.grad_timed++;
+      if (!fused && !use_graph) {
+        for (int k = 0; k < kCheck; ++k) {
+          float ms = 0;
+          HIPC(hipEventElapsedTime(&ms, ev.a[k], ev.b[k]));
+          res.grad_s += ms * 1e-3;
+          res.grad_timed++;
+        }
       }
       if (stopped >= C) break;
     }

@@ -246,6 +246,43 @@ def bench_c4(args):
             "speedup_total": out["flow"]["ess_per_sec_total"] / out["plain"]["ess_per_sec_total"]}
 
 
+def bench_c5(args):
+    """Logistic regression N=1,000,000 x D=1000 (X stored BF16 or FP32), 512
+    chains, dense_e, FP32 matrix-core X.B / R^T.X with FP64 sums."""
+    import cmdstan_amd as ca
+    from cmdstan_amd.summary import summarize
+    N, D, C = args.N5, args.D5, 512
+    rng = np.random.default_rng(args.seed)
+    print(f"c5: generating X {N}x{D}", file=sys.stderr, flush=True)
+    X = rng.standard_normal((N, D), dtype=np.float32).astype(np.float64)
+    beta = rng.normal(0, np.sqrt(1.0 / D), D)
+    y = (rng.random(N) < 1 / (1 + np.exp(-(X @ beta)))).astype(np.int64)
+    m = ca.Model("logistic_regression", dict(N=N, D=D, X=X, y=y))
+    del X
+    prec = "bf16" if args.precision == "fp64" else args.precision
+    print(f"c5: sampling ({prec})", file=sys.stderr, flush=True)
+    fit = m.sample(num_chains=C, num_warmup=args.warmup, num_samples=args.steps, seed=args.seed, metric="dense_e",
+                   precision=prec, refresh=25)
+    arr = np.stack(fit.draws)
+    params = np.concatenate([arr[:, :, 7:7 + D], arr[:, :, :1]], axis=2).transpose(2, 0, 1)
+    st = summarize(params)
+    ess = float(np.nanmin(st[:, 7]))
+    grads = float(arr[:, :, 4].sum())
+    t = fit.grad_seconds_per_launch
+    flops = 4.0 * N * D * fit.grad_chains_per_launch
+    roof = dict(bound="mfma", achieved=flops / t / 1e12, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                frac=flops / t / 1e12 / PEAK_FP32_TFLOPS, traffic=None, kernel="gl_eta_kernel + gl_grad_kernel",
+                launch_ms=t * 1e3, mean_active_chains=fit.grad_chains_per_launch) if t > 0 else None
+    return {"metric": "min bulk-ESS/sec, large logistic regression (dense_e)", "value": ess / fit.sampling_seconds,
+            "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True,
+            "dtype": f"f32 contraction (X {prec}), f64 sums", "data": "synthetic",
+            "config": {"workload": f"logistic regression N={N} D={D}, {C} chains, dense_e, X stored {prec}"},
+            "grad_evals_per_sec": grads / fit.sampling_seconds, "min_bulk_ess": ess,
+            "rhat_max": float(np.nanmax(st[:, 9])), "sampling_seconds": fit.sampling_seconds,
+            "warmup_seconds": fit.warmup_seconds, "roofline": roof}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,10 +295,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"],
                     help="GLM contraction precision (sums, H and tree arithmetic stay FP64)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default): the headline workload; c1, c3: secondary measurements")
     ap.add_argument("--models", type=int, default=512, help="c3: number of models")
     ap.add_argument("--flow-steps", type=int, default=1000, help="c4: Adam steps")
+    ap.add_argument("--N5", type=int, default=1000000, help="c5: observations")
+    ap.add_argument("--D5", type=int, default=1000, help="c5: coefficients")
     ap.add_argument("--flow-warmup", type=int, default=300, help="c4: warmup iterations on the flow")
     ap.add_argument("--flow-pilot", type=int, default=100, help="c4: pilot draws per sampler")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -282,6 +321,9 @@ def main():
         return
     if args.config == "c4":
         print(json.dumps(bench_c4(args)))
+        return
+    if args.config == "c5":
+        print(json.dumps(bench_c5(args)))
         return
 
     import cmdstan_amd as ca

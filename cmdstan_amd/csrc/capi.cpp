@@ -22,6 +22,7 @@
 #include "launch.h"
 #include "flow.h"
 #include "hier_logistic.h"
+#include "glm_large.h"
 #include "nuts_types.h"
 
 using namespace nutsgpu;
@@ -124,8 +125,9 @@ struct ModelImpl {
   std::vector<Dataset> ds;
   DevBuf<double> d_data;
   std::vector<int64_t> ds_off;
-  // logistic regression batched path (single dataset)
-  bool glm = false;
+  // logistic regression batched paths (single dataset): glm for D <= 128 (FP64
+  // or FP32/BF16 contraction), glm_large for larger D (FP32/BF16 contraction)
+  bool glm = false, glm_large = false;
   int glmN = 0, glmD = 0;
   double glm_s2inv = 0;
   const double* d_X = nullptr;
@@ -553,7 +555,9 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   const bool glm_ok = M.glm && nds == 1 && !R.flow;
   // hierarchical logistic: batched group kernel (lane = chain) over one dataset
   const bool hl_ok = M.kind == MK_HIER_LOGISTIC && nds == 1;
-  if (cfg.fused < 0) fused = !glm_ok && !hl_ok;
+  // large-D logistic regression: two-pass FP32 matrix-core gradient (X FP32 or BF16)
+  const bool gl_ok = M.glm_large && nds == 1 && !R.flow && cfg.precision != 0 && cfg.fused != 1;
+  if (cfg.fused < 0) fused = !glm_ok && !hl_ok && !gl_ok;
   if (R.flow) {
     if (nds != 1 || R.flow->dim() != D[0] || R.flow->max_rows() < C)
       throw std::invalid_argument("flow: one dataset whose dimension matches the flow, at most max_rows chains");
@@ -598,6 +602,20 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     if (hl_ok) {
       hla.Q = fX.p; hla.ldq = D[0]; hla.Gout = fGX.p; hla.ldg = D[0]; hla.Vout = fVX.p;
     }
+  }
+  GlLargeArgs gla;
+  GlLargePlan glp;
+  DevBuf<float> glQf, glR;
+  DevBuf<double> glWg, glWlp;
+  if (!fused && gl_ok) {
+    act_idx.alloc(C);
+    n_act.alloc(1);
+    glp = gl_large_plan(M.glmN, C, M.glmD, num_cus());
+    glQf.alloc(glp.q_floats); glR.alloc(glp.r_floats); glWg.alloc(glp.wg_doubles); glWlp.alloc(glp.wlp_doubles);
+    gla.y = M.d_y; gla.N = M.glmN; gla.D = M.glmD; gla.C = C; gla.s2inv = M.glm_s2inv;
+    gla.Q = Q.p; gla.ldq = Dp[0]; gla.G = G.p; gla.V = V.p; gla.idx = act_idx.p; gla.n_active = n_act.p;
+    gla.Qf = glQf.p; gla.Dq = glp.Dq; gla.R = glR.p; gla.ldr = glp.ldr; gla.Wg = glWg.p; gla.Wlp = glWlp.p;
+    xlow = M.low_x(cfg.precision, st);
   }
   if (!fused && glm_ok) {
     act_idx.alloc(C);
@@ -653,6 +671,11 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
                                            : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
         if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+      } else if (gl_ok) {
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+        gla.ev_begin = with_events ? ev.a[k] : nullptr;
+        gla.ev_end = with_events ? ev.b[k] : nullptr;
+        gl_large_grad(gla, glp, xlow, cfg.precision, st);
       } else if (R.flow) {
         // pulled-back gradient: x = T(z) for the active chains, model at x, VJP to z
         launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
@@ -679,7 +702,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   // launch-bound paths (flow, per-chain models), off for the GLM contraction,
   // whose kernel is timed with events.
   const char* genv = std::getenv("CMDSTAN_AMD_GRAPH");
-  const bool use_graph = !fused && (genv ? std::atoi(genv) != 0 : !glm_ok);
+  const bool use_graph = !fused && (genv ? std::atoi(genv) != 0 : !glm_ok && !gl_ok);
   struct GraphHold {
     hipGraph_t g = nullptr;
     hipGraphExec_t x = nullptr;
@@ -1046,8 +1069,9 @@ nuts_model* nuts_model_create(const char* family, const nuts_data* const* datase
     M.d_data.alloc(std::max<size_t>(packed.size(), 1));
     M.d_data.upload(packed.data(), packed.size());
     HIPC(hipDeviceSynchronize());
-    if (M.kind == MK_LOGISTIC && num_datasets == 1 && M.ds[0].D <= 128) {
-      M.glm = true;
+    if (M.kind == MK_LOGISTIC && num_datasets == 1) {
+      M.glm = M.ds[0].D <= 128;
+      M.glm_large = !M.glm;
       M.glmN = (int)packed[0];
       M.glmD = (int)packed[1];
       M.glm_s2inv = packed[2];
@@ -1107,9 +1131,18 @@ static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n
     DevBuf<double> Q(hq.size()), G(hq.size()), V(C);
     dD.upload(Dv); doff.upload(off); ddso.upload(dso); Q.upload(hq);
     ModelDataDev md{M.d_data.p, ddso.p, jacobian ? 1.0 : 0.0};
-    if (precision != 0 && !(M.glm && dataset == 0))
+    if (precision != 0 && !((M.glm || M.glm_large) && dataset == 0))
       throw std::invalid_argument("reduced precision is available for the batched logistic regression only");
-    if (M.glm && dataset == 0) {
+    if (M.glm_large && dataset == 0 && precision != 0) {
+      const GlLargePlan gp = gl_large_plan(M.glmN, C, D, num_cus());
+      DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
+      DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
+      GlLargeArgs a;
+      a.y = M.d_y; a.N = M.glmN; a.D = D; a.C = C; a.s2inv = M.glm_s2inv; a.Q = Q.p; a.ldq = Dp; a.G = G.p; a.V = V.p;
+      a.Qf = qf.p; a.Dq = gp.Dq; a.R = rr.p; a.ldr = gp.ldr; a.Wg = wg.p; a.Wlp = wl.p;
+      gl_large_grad(a, gp, M.low_x(precision, 0), precision, 0);
+      HIPC(hipDeviceSynchronize());
+    } else if (M.glm && dataset == 0) {
       const GlmPlan gp = glm_mfma_plan(M.glmN, C, D, num_cus());
       DevBuf<double> work(gp.work_doubles);
       GlmArgs ga{M.d_X, M.d_y, M.glmN, D, C, gp.rows_per_chunk, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};

@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(256) hl_group_kernel(HlArgs a) {
   }
 }
 
-// hyperparameters: lane k of the wave of position pos
+// hyperparameters, one wave per position: lanes take groups g = lane, lane+64,
+// ... and keep the 2K+1 column sums in registers, then combine with butterflies
 __global__ void __launch_bounds__(256) hl_hyper_kernel(HlArgs a) {
   const double* d = a.block;
   const int K = (int)d[1], G = (int)d[2];
@@ -117,26 +118,34 @@ __global__ void __launch_bounds__(256) hl_hyper_kernel(HlArgs a) {
   const double* q = a.Q + row_in * a.ldq;
   double* gout = a.Gout + row_out * a.ldg;
   const int W = 2 * K + 1;
-  // column j of the per-group partials, summed over groups in order
+  double acc[2 * HL_KMAX + 1];
+#pragma unroll
+  for (int j = 0; j < 2 * HL_KMAX + 1; ++j) acc[j] = 0.0;
+  for (int g = lane; g < G; g += NUTS_WAVE) {
+    const double* pr = a.work + ((int64_t)g * a.n_rows + pos) * W;
+#pragma unroll
+    for (int j = 0; j < 2 * HL_KMAX + 1; ++j)
+      if (j < W) acc[j] += pr[j];
+  }
   double lp = 0.0;
-  for (int j = lane; j < W; j += NUTS_WAVE) {
-    double s = 0.0;
-    for (int g = 0; g < G; ++g) s += a.work[((int64_t)g * a.n_rows + pos) * W + j];
+#pragma unroll
+  for (int j = 0; j < 2 * HL_KMAX + 1; ++j) {
+    if (j >= W) continue;
+    const double s = wave_sum(acc[j]);
     if (j < K) {
       const double mu = q[j];
-      gout[j] = -(s - mu / (mu_s * mu_s));
+      if (lane == 0) gout[j] = -(s - mu / (mu_s * mu_s));
       lp += -0.5 * mu * mu / (mu_s * mu_s);
     } else if (j < 2 * K) {
       const double v = q[j];
       const double tau = exp(v);
-      gout[j] = -(a.jac + tau * (s - tau));
+      if (lane == 0) gout[j] = -(a.jac + tau * (s - tau));
       lp += -0.5 * tau * tau + a.jac * v;
     } else {
       lp += s;
     }
   }
-  lp = wave_sum(lp);
-  if (lane == 0) a.Vout[a.idx_out ? a.idx[pos] : pos] = -lp;
+  if (lane == 0) a.Vout[row_out] = -lp;
 }
 
 void hl_grad_launch(const HlArgs& a, int G, hipStream_t st) {

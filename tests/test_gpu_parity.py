@@ -398,3 +398,46 @@ def test_graph_replay_matches_eager_with_flow(monkeypatch):
     assert a.flow_used == b.flow_used
     for x, y in zip(a.draws, b.draws):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_glm_large_gradient_reduced_precision(precision):
+    """Large-D path (two FP32 matrix-core passes, X in FP32 or stored BF16,
+    FP64 sums of the row-range partials): error against FP64 on the same stored
+    data relative to the sum of |terms|, tolerance 1e-5; ragged N, D, C."""
+    N, D, C = 6001, 301, 70
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal((N, D))
+    beta = rng.normal(0, 1 / np.sqrt(D), D)
+    y = (rng.random(N) < 1 / (1 + np.exp(-X @ beta))).astype(np.int64)
+    gm = ca.Model("logistic_regression", dict(N=N, D=D, X=X, y=y))
+    Q = beta + rng.normal(0, 0.05, (C, D))
+    lp, g = gm.log_prob_grad(Q, precision=precision)
+    Xs = X.astype(np.float32).astype(np.float64) if precision == "fp32" else _bf16_round(X)
+    E_ = Xs @ Q.T
+    R = y[:, None] - 1 / (1 + np.exp(-E_))
+    g_ref = (Xs.T @ R).T - Q / 6.25
+    lp_ref = (y[:, None] * E_ - np.logaddexp(0, E_)).sum(0) - 0.5 * (Q * Q).sum(1) / 6.25
+    cond_g = (np.abs(Xs).T @ np.abs(R)).T + np.abs(Q) / 6.25
+    cond_lp = np.abs(y[:, None] * E_).sum(0) + np.logaddexp(0, E_).sum(0)
+    err_g = np.max(np.abs(g - g_ref) / cond_g)
+    err_lp = np.max(np.abs(lp - lp_ref) / cond_lp)
+    print(f"{precision}: max rel gradient error {err_g:.2e}, lp error {err_lp:.2e}")
+    assert err_g < 1e-5 and err_lp < 1e-5
+
+
+def test_glm_large_dense_posterior_matches_fp64():
+    """dense_e sampling through the large-D FP32 path agrees with the FP64
+    per-chain path (means within 4 combined MCSE)."""
+    N, D = 3000, 150
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((N, D))
+    beta = rng.normal(0, 1 / np.sqrt(D), D)
+    y = (rng.random(N) < 1 / (1 + np.exp(-X @ beta))).astype(np.int64)
+    m = ca.Model("logistic_regression", dict(N=N, D=D, X=X, y=y))
+    a = m.sample(num_chains=64, num_warmup=300, num_samples=200, seed=4, metric="dense_e", precision="fp32")
+    b = m.sample(num_chains=64, num_warmup=300, num_samples=200, seed=5, metric="dense_e")
+    assert np.all(a.status == 0) and np.all(b.status == 0)
+    for j in (1, 50, 150):
+        sa, sb = E.summary(a.param(f"beta.{j}")), E.summary(b.param(f"beta.{j}"))
+        assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"]), (j, sa, sb)

@@ -356,7 +356,8 @@ def main():
         flops_per_launch = 4.0 * args.N * D * fit.grad_chains_per_launch
         ach = flops_per_launch / t / 1e12
         peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
-        kname = "glm_mfma_f64" if args.precision == "fp64" else "glm_mfma_f32"
+        kname = ("glm_mfma_f64" if args.precision == "fp64" else
+                 "glm_mfma_f32" if fit.tuned_path == 0 else "gl_eta_kernel + gl_grad_kernel")
         roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
                     traffic=measured_traffic(kname, args), kernel=kname, launch_ms=t * 1e3,
                     launches=fit.grad_launches_timed, mean_active_chains=fit.grad_chains_per_launch,
@@ -388,6 +389,8 @@ def main():
         "warmup_seconds": fit.warmup_seconds,
         "total_wall_seconds": wall,
         "mean_n_leapfrog": nlf_samp / (C * args.steps),
+        "autotuned_gradient": {"path": ["single-pass", "two-pass"][fit.tuned_path] if fit.tuned_path >= 0 else None,
+                               "blocks_per_cu": fit.tuned_blocks_per_cu},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

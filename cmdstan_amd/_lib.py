@@ -35,6 +35,7 @@ class nuts_output(C.Structure):
         ("grad_chains_per_launch", C.c_double),
         ("flow_used", C.c_int), ("flow_train_seconds", C.c_double), ("flow_loss", C.c_double),
         ("pilot_ess_per_grad_plain", C.c_double), ("pilot_ess_per_grad_flow", C.c_double),
+        ("tuned_path", C.c_int), ("tuned_blocks_per_cu", C.c_int),
     ]
 
 

@@ -132,6 +132,10 @@ struct ModelImpl {
   double glm_s2inv = 0;
   const double* d_X = nullptr;
   const double* d_y = nullptr;
+  // autotuned GLM gradient per (chains, precision): path 0 = single-pass
+  // kernel with blocks_per_cu resident blocks per CU, path 1 = two-pass kernels
+  struct GlmTune { int path = 0, blocks_per_cu = 2; double ms = 0; };
+  std::map<std::pair<int, int>, GlmTune> glm_tune;
   DevBuf<unsigned char> xlow;  // X converted to FP32 or BF16 (lazily)
   int xlow_prec = 0;
   const void* low_x(int precision, hipStream_t st) {
@@ -327,6 +331,7 @@ struct RunResult {
   double grad_s = 0;
   int64_t grad_timed = 0;
   long long grad_chains = 0;  // chains evaluated, summed over GLM gradient launches
+  int tuned_path = -1, tuned_bpc = 0;
   std::vector<double> z_last;  // [C][D] unconstrained q of each chain's last saved row (before any flow map)
   int rows = 0;
 };
@@ -386,6 +391,60 @@ __global__ void min_iter_kernel(const ChainState* st, int C, int* out) {
 __global__ void resume_kernel(ChainState* st, int C) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < C && st[c].phase == PH_PAUSED) st[c].phase = PH_TRANS_BEGIN;
+}
+
+// Times the candidate GLM gradient implementations on the model's own data with
+// every chain active (positions at zero) and keeps the fastest per
+// (chains, precision).  Candidates: the single-pass kernel at 1-4 resident
+// blocks per CU, and for FP32/BF16 the two-pass kernels.
+ModelImpl::GlmTune glm_autotune(ModelImpl& M, int C, int precision, hipStream_t st) {
+  const auto key = std::make_pair(C, precision);
+  auto it = M.glm_tune.find(key);
+  if (it != M.glm_tune.end()) return it->second;
+  const int N = M.glmN, D = M.glmD, Dp = (D + 15) / 16 * 16;
+  DevBuf<double> Q((size_t)C * Dp), G((size_t)C * Dp), V(C);
+  HIPC(hipMemsetAsync(Q.p, 0, (size_t)C * Dp * sizeof(double), st));
+  const void* xl = precision ? M.low_x(precision, st) : nullptr;
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  auto time_it = [&](auto&& launch) {
+    launch();
+    HIPC(hipEventRecord(e0, st));
+    for (int r = 0; r < 3; ++r) launch();
+    HIPC(hipEventRecord(e1, st));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    return ms / 3.0;
+  };
+  ModelImpl::GlmTune best;
+  best.ms = 1e30;
+  for (int bpc = 1; bpc <= 4; ++bpc) {
+    const GlmPlan gp = glm_mfma_plan(N, C, D, num_cus(), bpc);
+    DevBuf<double> work(gp.work_doubles);
+    GlmArgs ga{M.d_X, M.d_y, N, D, C, gp.rows_per_chunk, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
+    const double ms = time_it([&]() {
+      if (precision == 0) glm_grad_mfma_f64(ga, gp, st);
+      else glm_grad_mfma_f32(ga, gp, xl, precision, kF32Layout, st);
+    });
+    if (ms < best.ms) { best.ms = ms; best.path = 0; best.blocks_per_cu = bpc; }
+  }
+  if (precision != 0) {
+    const GlLargePlan gp = gl_large_plan(N, C, D, num_cus());
+    DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
+    DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
+    GlLargeArgs a;
+    a.y = M.d_y; a.N = N; a.D = D; a.C = C; a.s2inv = M.glm_s2inv; a.Q = Q.p; a.ldq = Dp; a.G = G.p; a.V = V.p;
+    a.Qf = qf.p; a.Dq = gp.Dq; a.R = rr.p; a.ldr = gp.ldr; a.Wg = wg.p; a.Wlp = wl.p;
+    const double ms = time_it([&]() { gl_large_grad(a, gp, xl, precision, st); });
+    if (ms < best.ms) { best.ms = ms; best.path = 1; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPC(hipStreamSynchronize(st));
+  M.glm_tune[key] = best;
+  return best;
 }
 
 RunResult run_batch(ModelImpl& M, const RunSpec& R) {
@@ -552,11 +611,23 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   ModelDataDev md{M.d_data.p, ddsoff.p, 1.0};
 
   bool fused = cfg.fused == 1;
-  const bool glm_ok = M.glm && nds == 1 && !R.flow;
+  // FP32/BF16 contraction of a small-D model through the two-pass large-D kernels
+  // (selected by the autotuner, or forced with CMDSTAN_AMD_GLM_LARGE=0/1)
+  const char* gle = std::getenv("CMDSTAN_AMD_GLM_LARGE");
+  ModelImpl::GlmTune tune;
+  int res_tuned_path = -1, res_tuned_bpc = 0;
+  if (M.glm && nds == 1 && !R.flow && cfg.fused != 1) tune = glm_autotune(M, C, cfg.precision, st);
+  const bool small_as_large =
+      M.glm && cfg.precision != 0 && (gle ? std::atoi(gle) != 0 : tune.path == 1);
+  const bool glm_ok = M.glm && nds == 1 && !R.flow && !small_as_large;
+  if (M.glm && nds == 1 && !R.flow && cfg.fused != 1) {
+    res_tuned_path = small_as_large ? 1 : 0;
+    res_tuned_bpc = tune.blocks_per_cu;
+  }
   // hierarchical logistic: batched group kernel (lane = chain) over one dataset
   const bool hl_ok = M.kind == MK_HIER_LOGISTIC && nds == 1;
   // large-D logistic regression: two-pass FP32 matrix-core gradient (X FP32 or BF16)
-  const bool gl_ok = M.glm_large && nds == 1 && !R.flow && cfg.precision != 0 && cfg.fused != 1;
+  const bool gl_ok = (M.glm_large || small_as_large) && nds == 1 && !R.flow && cfg.precision != 0 && cfg.fused != 1;
   if (cfg.fused < 0) fused = !glm_ok && !hl_ok && !gl_ok;
   if (R.flow) {
     if (nds != 1 || R.flow->dim() != D[0] || R.flow->max_rows() < C)
@@ -620,7 +691,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   if (!fused && glm_ok) {
     act_idx.alloc(C);
     n_act.alloc(1);
-    gplan = glm_mfma_plan(M.glmN, C, M.glmD, num_cus());
+    gplan = glm_mfma_plan(M.glmN, C, M.glmD, num_cus(), tune.blocks_per_cu);
     work.alloc(gplan.work_doubles);
     ga.X = M.d_X; ga.y = M.d_y; ga.N = M.glmN; ga.D = M.glmD; ga.C = C; ga.rows_per_chunk = gplan.rows_per_chunk;
     ga.s2inv = M.glm_s2inv; ga.Q = Q.p; ga.ldq = Dp[0]; ga.G = G.p; ga.V = V.p; ga.work = work.p;
@@ -630,6 +701,8 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
 
   RunResult res;
   res.rows = rows;
+  res.tuned_path = res_tuned_path;
+  res.tuned_bpc = res_tuned_bpc;
   int stopped = 0;
   // one event pair per launch of a check interval brackets the gradient's
   // contraction kernel (the whole gradient kernel off the GLM path)
@@ -1237,6 +1310,8 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
       out->grad_seconds_per_launch = r.grad_timed ? r.grad_s / r.grad_timed : 0.0;
       out->grad_launches_timed = r.grad_timed;
       out->grad_chains_per_launch = r.leaf_steps ? (double)r.grad_chains / r.leaf_steps : 0.0;
+      out->tuned_path = r.tuned_path;
+      out->tuned_blocks_per_cu = r.tuned_bpc;
       if (cfg->flow) {
         out->warmup_seconds = rep.pre_s;  // plain warmup + pilot, training, flow warmup + pilot
         out->flow_used = rep.used;

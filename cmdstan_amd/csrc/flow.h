@@ -76,10 +76,10 @@ class FlowNet {
     int* ia; int* ib;          // device index lists
     int64_t off[6];            // W1 b1 W2 b2 W3 b3 offsets in the flat parameter vector
   };
-  void mlp(int l, const float* UA, const int* n_dev, int rows);
+  void mlp(int l, const float* U, int ldu, const int* ia, const int* n_dev, int rows);
   void gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn, float* C,
             int64_t ldc, int M, int N, int K, const int* Mdev, int epi, const float* bias, const float* aux,
-            int64_t ldaux);
+            int64_t ldaux, const int* aidx_k = nullptr, const int* aidx_m = nullptr, const int* cidx_n = nullptr);
   void inverse_pass(int rows);     // U_[L] -> U_[0], keeping the training tape
   void row_losses(int rows);       // rowbuf_ = per-row loss after inverse_pass
   void backward_params(int rows);  // parameter gradient of the mean loss into grad_

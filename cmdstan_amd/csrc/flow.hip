@@ -20,7 +20,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 #define FG_BM 64
 #define FG_BN 64
 #define FG_BK 16
-enum { FG_EPI_NONE = 0, FG_EPI_BIAS = 1, FG_EPI_BIAS_TANH = 2, FG_EPI_DTANH = 3 };
+enum { FG_EPI_NONE = 0, FG_EPI_BIAS = 1, FG_EPI_BIAS_TANH = 2, FG_EPI_DTANH = 3, FG_EPI_ADD = 4 };
 
 struct GemmDesc {
   const float* A; int64_t sam, sak;  // A(m, k) = A[m*sam + k*sak]
@@ -32,7 +32,14 @@ struct GemmDesc {
   const float* bias;                 // [N]
   const float* aux; int64_t ldaux;   // FG_EPI_DTANH: times (1 - aux(m, n)^2)
   float* part; int splits, kper;     // split-K partials [splits][M][N], kper rows of K per split
+  const int* aidx_k;                 // optional column gather of A: A(m, k) = A[m*sam + aidx_k[k]*sak]
+  const int* aidx_m;                 // optional row gather of A:    A(m, k) = A[aidx_m[m]*sam + k*sak]
+  const int* cidx_n;                 // optional column scatter of C: C(m, n) = C[m*ldc + cidx_n[n]]
 };
+
+__device__ __forceinline__ int64_t fg_cidx(const GemmDesc& g, int m, int n) {
+  return (int64_t)m * g.ldc + (g.cidx_n ? g.cidx_n[n] : n);
+}
 
 __device__ __forceinline__ float fg_epi(const GemmDesc& g, float v, int m, int n) {
   if (g.epi == FG_EPI_BIAS) return v + g.bias[n];
@@ -41,6 +48,7 @@ __device__ __forceinline__ float fg_epi(const GemmDesc& g, float v, int m, int n
     const float h = g.aux[(int64_t)m * g.ldaux + n];
     return v * (1.0f - h * h);
   }
+  if (g.epi == FG_EPI_ADD) return g.C[fg_cidx(g, m, n)] + v;
   return v;
 }
 
@@ -71,7 +79,13 @@ __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
       const int mm = a_k ? (t >> 2) : ((t & 15) * 4 + e);
       const int kk = a_k ? ((t & 3) * 4 + e) : (t >> 4);
       const int m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < M && k < ke) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.0f;
+      float v = 0.0f;
+      if (m < M && k < ke) {
+        const int64_t mr = g.aidx_m ? g.aidx_m[m] : m;
+        const int64_t kc = g.aidx_k ? g.aidx_k[k] : k;
+        v = g.A[mr * g.sam + kc * g.sak];
+      }
+      As[kk][mm] = v;
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -104,7 +118,7 @@ __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
         if (m < M && n < g.N) {
           const float v = acc[i][j][r];
           if (g.splits > 1) g.part[((int64_t)split * g.M + m) * g.N + n] = v;
-          else g.C[(int64_t)m * g.ldc + n] = fg_epi(g, v, m, n);
+          else g.C[fg_cidx(g, m, n)] = fg_epi(g, v, m, n);
         }
       }
 }
@@ -116,7 +130,7 @@ __global__ void flow_gemm_reduce(GemmDesc g) {
   const int m = (int)(i / g.N), n = (int)(i % g.N);
   float s = 0.0f;
   for (int sp = 0; sp < g.splits; ++sp) s += g.part[((int64_t)sp * g.M + m) * g.N + n];
-  g.C[(int64_t)m * g.ldc + n] = fg_epi(g, s, m, n);
+  g.C[fg_cidx(g, m, n)] = fg_epi(g, s, m, n);
 }
 
 // ------------------------------------------------------------ element-wise
@@ -179,20 +193,26 @@ __global__ void flow_couple_inv(const float* __restrict__ V, const float* __rest
   }
 }
 
-// sum over layers of sum_j s, plus sum log scale (FP64), one wave per row
-__global__ void flow_logdet(const float* __restrict__ S, int64_t layer_stride, const int* __restrict__ Db, int L,
-                            const int* __restrict__ n_dev, int n, double log_scale_sum, double* __restrict__ logdet,
-                            int sign) {
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+// sum over layers of sum_j s, plus sum log scale (FP64), one block per row
+__global__ void __launch_bounds__(256) flow_logdet(const float* __restrict__ S, int64_t layer_stride,
+                                                   const int* __restrict__ Db, int L, const int* __restrict__ n_dev,
+                                                   int n, double log_scale_sum, double* __restrict__ logdet, int sign) {
+  __shared__ double red[4];
+  const int r = blockIdx.x;
   if (r >= rows_of(n_dev, n)) return;
   double acc = 0;
   for (int l = 0; l < L; ++l) {
     const int db = Db[l];
     const float* s = S + l * layer_stride + (int64_t)r * db;
-    for (int j = lane; j < db; j += NUTS_WAVE) acc += (double)s[j];
+    for (int j = threadIdx.x; j < db; j += 256) acc += (double)s[j];
   }
   acc = wave_sum(acc);
-  if (lane == 0) logdet[r] = sign > 0 ? acc + log_scale_sum : -acc - log_scale_sum;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = red[0] + red[1] + red[2] + red[3];
+    logdet[r] = sign > 0 ? t + log_scale_sum : -t - log_scale_sum;
+  }
 }
 
 // X = loc + scale * U (FP64 out)
@@ -518,10 +538,11 @@ void FlowNet::set_standardization(const double* loc, const double* scale) {
 
 void FlowNet::gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn, float* C,
                    int64_t ldc, int M, int N, int K, const int* Mdev, int epi, const float* bias, const float* aux,
-                   int64_t ldaux) {
+                   int64_t ldaux, const int* aidx_k, const int* aidx_m, const int* cidx_n) {
   GemmDesc g{};
   g.A = A; g.sam = sam; g.sak = sak; g.B = B; g.sbk = sbk; g.sbn = sbn; g.C = C; g.ldc = ldc;
   g.M = M; g.N = N; g.K = K; g.Mdev = Mdev; g.epi = epi; g.bias = bias; g.aux = aux; g.ldaux = ldaux;
+  g.aidx_k = aidx_k; g.aidx_m = aidx_m; g.cidx_n = cidx_n;
   const int tm = (M + FG_BM - 1) / FG_BM, tn = (N + FG_BN - 1) / FG_BN;
   const int kblocks = (K + FG_BK - 1) / FG_BK;
   // split K until about 512 blocks, keeping >= 4 K-tiles per split
@@ -538,13 +559,14 @@ void FlowNet::gemm(const float* A, int64_t sam, int64_t sak, const float* B, int
 }
 
 // MLP of layer l on UA (rows x Da): H1, H2 of the layer, O = [s_raw, t]
-void FlowNet::mlp(int l, const float* UA, const int* n_dev, int rows) {
+void FlowNet::mlp(int l, const float* U, int ldu, const int* ia, const int* n_dev, int rows) {
   const Layer& ly = layers_[l];
   const int H = spec_.H;
   const float* P = params_;
   float* h1 = H1_ + (int64_t)l * R_ * H;
   float* h2 = H2_ + (int64_t)l * R_ * H;
-  gemm(UA, ly.Da, 1, P + ly.off[0], H, 1, h1, H, rows, H, ly.Da, n_dev, FG_EPI_BIAS_TANH, P + ly.off[1], nullptr, 0);
+  // the conditioner columns ia of U are gathered by the first product's loads
+  gemm(U, ldu, 1, P + ly.off[0], H, 1, h1, H, rows, H, ly.Da, n_dev, FG_EPI_BIAS_TANH, P + ly.off[1], nullptr, 0, ia);
   gemm(h1, H, 1, P + ly.off[2], H, 1, h2, H, rows, H, H, n_dev, FG_EPI_BIAS_TANH, P + ly.off[3], nullptr, 0);
   gemm(h2, H, 1, P + ly.off[4], 2 * ly.Db, 1, O_, 2 * ly.Db, rows, 2 * ly.Db, H, n_dev, FG_EPI_BIAS, P + ly.off[5],
        nullptr, 0);
@@ -559,14 +581,12 @@ void FlowNet::forward(const double* Q, int64_t ldq, const int* idx, const int* n
   for (int l = 0; l < L; ++l) {
     const Layer& ly = layers_[l];
     float* U = U_ + l * RD;
-    hipLaunchKernelGGL(flow_gather_cols, grid1((int64_t)n_rows * ly.Da), dim3(256), 0, st_, U, D, ly.ia, ly.Da, n_dev,
-                       n_rows, UA_);
-    mlp(l, UA_, n_dev, n_rows);
+    mlp(l, U, D, ly.ia, n_dev, n_rows);
     hipLaunchKernelGGL(flow_couple_fwd, grid1((int64_t)n_rows * D), dim3(256), 0, st_, U, O_, ly.ia, ly.ib, ly.Da,
                        ly.Db, D, n_dev, n_rows, S_ + (int64_t)l * R_ * Dmax_b_, U + RD);
   }
-  hipLaunchKernelGGL(flow_logdet, dim3((n_rows + 3) / 4), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_, L,
-                     n_dev, n_rows, log_scale_sum_, logdet, 1);
+  hipLaunchKernelGGL(flow_logdet, dim3(n_rows), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_, L, n_dev,
+                     n_rows, log_scale_sum_, logdet, 1);
   hipLaunchKernelGGL(flow_to_x, grid1((int64_t)n_rows * D), dim3(256), 0, st_, U_ + L * RD, loc_, scale_, D, n_dev,
                      n_rows, X);
 }
@@ -587,9 +607,9 @@ void FlowNet::pullback(const double* GX, const double* VX, const double* logdet,
     gemm(GO_, 2 * ly.Db, 1, P + ly.off[4], 1, 2 * ly.Db, GP2_, H, n_rows, H, 2 * ly.Db, n_dev, FG_EPI_DTANH, nullptr,
          h2, H);
     gemm(GP2_, H, 1, P + ly.off[2], 1, H, GP1_, H, n_rows, H, H, n_dev, FG_EPI_DTANH, nullptr, h1, H);
-    gemm(GP1_, H, 1, P + ly.off[0], 1, H, GUA_, ly.Da, n_rows, ly.Da, H, n_dev, FG_EPI_NONE, nullptr, nullptr, 0);
-    hipLaunchKernelGGL(flow_scatter_add, grid1((int64_t)n_rows * ly.Da), dim3(256), 0, st_, GC_, GUA_, ly.ia, ly.Da, D,
-                       n_dev, n_rows);
+    // GC[:, ia] += GP1 W1^T
+    gemm(GP1_, H, 1, P + ly.off[0], 1, H, GC_, D, n_rows, ly.Da, H, n_dev, FG_EPI_ADD, nullptr, nullptr, 0, nullptr,
+         nullptr, ly.ia);
   }
   hipLaunchKernelGGL(flow_finish, grid1((int64_t)n_rows * D), dim3(256), 0, st_, GC_, VX, logdet, idx, n_dev, n_rows,
                      D, G, ldg, V);
@@ -603,9 +623,7 @@ void FlowNet::inverse_pass(int rows) {
   for (int l = L - 1; l >= 0; --l) {
     const Layer& ly = layers_[l];
     const float* Vin = U_ + (l + 1) * RD;
-    hipLaunchKernelGGL(flow_gather_cols, grid1((int64_t)rows * ly.Da), dim3(256), 0, st_, Vin, D, ly.ia, ly.Da,
-                       nullptr, rows, UA_);
-    mlp(l, UA_, nullptr, rows);
+    mlp(l, Vin, D, ly.ia, nullptr, rows);
     hipLaunchKernelGGL(flow_couple_inv, grid1((int64_t)rows * D), dim3(256), 0, st_, Vin, O_, ly.ia, ly.ib, ly.Da,
                        ly.Db, D, rows, S_ + (int64_t)l * R_ * Dmax_b_, U_ + l * RD);
   }
@@ -613,8 +631,8 @@ void FlowNet::inverse_pass(int rows) {
 
 // Row losses 0.5|z|^2 - log|det dT^{-1}/dx| into rowbuf_ (rows) after inverse_pass().
 void FlowNet::row_losses(int rows) {
-  hipLaunchKernelGGL(flow_logdet, dim3((rows + 3) / 4), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_,
-                     spec_.L, nullptr, rows, log_scale_sum_, rowbuf2_, -1);
+  hipLaunchKernelGGL(flow_logdet, dim3(rows), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_, spec_.L, nullptr,
+                     rows, log_scale_sum_, rowbuf2_, -1);
   hipLaunchKernelGGL(flow_half_sq, dim3((rows + 3) / 4), dim3(256), 0, st_, U_, spec_.D, rows, rowbuf_);
   hipLaunchKernelGGL(flow_loss_rows, grid1(rows), dim3(256), 0, st_, rowbuf_, rowbuf2_, rows);
 }
@@ -642,14 +660,13 @@ void FlowNet::backward_params(int rows) {
     hipLaunchKernelGGL(flow_colsum, grid1(H), dim3(256), 0, st_, GP2_, rows, H, G + ly.off[3]);
     // GP1 = (GP2 W2^T)(1 - h1^2); W1 from the layer's conditioner input, b1
     gemm(GP2_, H, 1, P + ly.off[2], 1, H, GP1_, H, rows, H, H, nullptr, FG_EPI_DTANH, nullptr, h1, H);
-    hipLaunchKernelGGL(flow_gather_cols, grid1((int64_t)rows * ly.Da), dim3(256), 0, st_, U_ + (l + 1) * RD, D, ly.ia,
-                       ly.Da, nullptr, rows, UA_);
-    gemm(UA_, 1, ly.Da, GP1_, H, 1, G + ly.off[0], H, ly.Da, H, rows, nullptr, FG_EPI_NONE, nullptr, nullptr, 0);
+    // W1: A(j, r) = U_[l+1][r][ia[j]] (row gather on the conditioner columns)
+    gemm(U_ + (l + 1) * RD, 1, D, GP1_, H, 1, G + ly.off[0], H, ly.Da, H, rows, nullptr, FG_EPI_NONE, nullptr, nullptr,
+         0, nullptr, ly.ia);
     hipLaunchKernelGGL(flow_colsum, grid1(H), dim3(256), 0, st_, GP1_, rows, H, G + ly.off[1]);
-    // cotangent of the conditioner input
-    gemm(GP1_, H, 1, P + ly.off[0], 1, H, GUA_, ly.Da, rows, ly.Da, H, nullptr, FG_EPI_NONE, nullptr, nullptr, 0);
-    hipLaunchKernelGGL(flow_scatter_add, grid1((int64_t)rows * ly.Da), dim3(256), 0, st_, GC_, GUA_, ly.ia, ly.Da, D,
-                       nullptr, rows);
+    // cotangent of the conditioner input: GC[:, ia] += GP1 W1^T
+    gemm(GP1_, H, 1, P + ly.off[0], 1, H, GC_, D, rows, ly.Da, H, nullptr, FG_EPI_ADD, nullptr, nullptr, 0, nullptr,
+         nullptr, ly.ia);
   }
 }
 
@@ -668,8 +685,8 @@ void FlowNet::inverse(const double* X, int n_rows, double* Z, double* logdet) {
   inverse_pass(n_rows);
   hipLaunchKernelGGL(flow_rows_out, grid1((int64_t)n_rows * D), dim3(256), 0, st_, U_, (int64_t)n_rows * D, Z);
   if (logdet)
-    hipLaunchKernelGGL(flow_logdet, dim3((n_rows + 3) / 4), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_, L,
-                       nullptr, n_rows, log_scale_sum_, logdet, -1);
+    hipLaunchKernelGGL(flow_logdet, dim3(n_rows), dim3(256), 0, st_, S_, (int64_t)R_ * Dmax_b_, db_dev_, L, nullptr,
+                       n_rows, log_scale_sum_, logdet, -1);
 }
 
 double FlowNet::loss_grad(const double* X, int n_rows, float* grad_host) {

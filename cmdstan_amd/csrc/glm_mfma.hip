@@ -224,11 +224,12 @@ void launch_compact_active(int* need, int C, int* idx, int* n_active, long long*
   hipLaunchKernelGGL(compact_active, dim3(1), dim3(1024), 0, st, need, C, idx, n_active, tally);
 }
 
-GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus) {
+GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus, int blocks_per_cu) {
   GlmPlan p;
-  // about two resident blocks per CU, and at least 8 row chunks per chain group
+  // blocks_per_cu resident blocks per CU (2 unless tuned), and at least 8 row
+  // chunks per chain group
   const int groups = (C + GM_TC - 1) / GM_TC;
-  int tb = 2 * (num_cus > 0 ? num_cus : 256);
+  int tb = (blocks_per_cu > 0 ? blocks_per_cu : 2) * (num_cus > 0 ? num_cus : 256);
   if (tb < 8 * groups) tb = 8 * groups;
   p.total_blocks = (tb + 7) & ~7;
   const GlmSplit full = glm_split(C, N, p.total_blocks);

@@ -58,19 +58,27 @@ __global__ void __launch_bounds__(256) hl_group_kernel(HlArgs a) {
     for (int i = t; i < cnt * K; i += 256) sX[i] = X[(int64_t)base * K + i];
     for (int i = t; i < cnt; i += 256) sY[i] = y[base + i];
     __syncthreads();
-    for (int n = w; n < cnt; n += 4) {
+    // two observations per step: independent exp/log chains in flight
+    for (int n = w; n < cnt; n += 8) {
+      const int n2 = n + 4;
+      const bool two = n2 < cnt;
       const double* xr = sX + n * K;
-      double e = 0.0;
+      const double* xr2 = sX + (two ? n2 : n) * K;
+      double e = 0.0, e2 = 0.0;
 #pragma unroll
       for (int k = 0; k < HL_KMAX; ++k)
-        if (k < K) e += xr[k] * b[k];
-      double sig;
-      const double yy = sY[n];
-      lp += yy * e - hl_log1p_exp(e, &sig);
+        if (k < K) { e += xr[k] * b[k]; e2 += xr2[k] * b[k]; }
+      double sig, sig2;
+      const double yy = sY[n], yy2 = two ? sY[n2] : 0.0;
+      const double l1 = hl_log1p_exp(e, &sig);
+      const double l2 = hl_log1p_exp(e2, &sig2);
+      lp += yy * e - l1;
       const double r = yy - sig;
+      const double r2 = two ? yy2 - sig2 : 0.0;
+      if (two) lp += yy2 * e2 - l2;
 #pragma unroll
       for (int k = 0; k < HL_KMAX; ++k)
-        if (k < K) gb[k] += r * xr[k];
+        if (k < K) gb[k] += r * xr[k] + r2 * xr2[k];
     }
   }
   // combine the four waves' partial sums

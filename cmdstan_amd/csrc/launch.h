@@ -65,7 +65,7 @@ struct GlmPlan {
   int total_blocks = 0;                               // grid of every launch (the split adapts to the active count)
   size_t work_doubles = 0;
 };
-GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus);
+GlmPlan glm_mfma_plan(int N, int C, int D, int num_cus, int blocks_per_cu = 2);
 // tally (optional, device) accumulates the number of active chains over launches.
 void launch_compact_active(int* need, int C, int* idx, int* n_active, long long* tally, hipStream_t st);
 void glm_reduce_launch(const GlmArgs& a, const GlmPlan& p, hipStream_t st);

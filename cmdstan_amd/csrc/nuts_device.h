@@ -254,8 +254,12 @@ struct NutsWave {
   }
 
   // ------------------------------------------------------------- vectors
-  __device__ void copy(double* dst, const double* src) const {
-    for (int i = tid; i < D; i += TPC) dst[i] = src[i]; converge();
+  // the pool vectors never alias each other: restrict lets the unrolled loops
+  // issue several independent loads before their stores
+  __device__ void copy(double* __restrict__ dst, const double* __restrict__ src) const {
+#pragma unroll 4
+    for (int i = tid; i < D; i += TPC) dst[i] = src[i];
+    converge();
   }
   __device__ void swap(double* a, double* b) const {
     for (int i = tid; i < D; i += TPC) { double t = a[i]; a[i] = b[i]; b[i] = t; } converge();
@@ -271,17 +275,25 @@ struct NutsWave {
       for (int i = tid; i < D; i += TPC) q[i] = q[i] + e * t[i]; converge();
       return;
     }
+    double* __restrict__ pr = p;
+    double* __restrict__ qr = q;
+    const double* __restrict__ gr = g;
+#pragma unroll 4
     for (int i = tid; i < D; i += TPC) {
-      const double pi = p[i] - h * g[i];
-      p[i] = pi;
-      q[i] = q[i] + e * (mi(i) * pi);
-    } converge();
+      const double pi = pr[i] - h * gr[i];
+      pr[i] = pi;
+      qr[i] = qr[i] + e * (mi(i) * pi);
+    }
+    converge();
   }
   // end_update_p(eps/2); dense: also the leaf's sharp momentum
   __device__ void leap_end(double e) const {
-    double* p = vecp(PV_P);
+    double* __restrict__ p = vecp(PV_P);
+    const double* __restrict__ gr = g;
     const double h = 0.5 * e;
-    for (int i = tid; i < D; i += TPC) p[i] = p[i] - h * g[i]; converge();
+#pragma unroll 4
+    for (int i = tid; i < D; i += TPC) p[i] = p[i] - h * gr[i];
+    converge();
     if (DENSE) matvec(p, vecp(PV_PS));
   }
 

@@ -198,7 +198,8 @@ class Model:
                   grad_chains_per_launch=out.grad_chains_per_launch, flow_used=bool(out.flow_used),
                   flow_train_seconds=out.flow_train_seconds, flow_loss=out.flow_loss,
                   pilot_ess_per_grad_plain=out.pilot_ess_per_grad_plain,
-                  pilot_ess_per_grad_flow=out.pilot_ess_per_grad_flow, column_names=names,
+                  pilot_ess_per_grad_flow=out.pilot_ess_per_grad_flow, tuned_path=out.tuned_path,
+                  tuned_blocks_per_cu=out.tuned_blocks_per_cu, column_names=names,
                   dataset_of_chain=ds_of_chain, config=cfg, error=err)
         if rc != 0:
             raise SamplerError(err)

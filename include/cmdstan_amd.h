@@ -167,6 +167,11 @@ typedef struct nuts_output {
   double flow_loss;               /* mean negative log-likelihood of the last training batches */
   double pilot_ess_per_grad_plain;
   double pilot_ess_per_grad_flow;
+  /* batched GLM gradient chosen by the autotuner (lockstep logistic regression):
+     path 0 single-pass kernel with tuned_blocks_per_cu blocks per CU, 1 two-pass
+     kernels; -1 when not applicable */
+  int tuned_path;
+  int tuned_blocks_per_cu;
 } nuts_output;
 
 /* Runs warmup then sampling for every chain of every dataset in one batch.

@@ -214,10 +214,12 @@ def bench_c4(args):
         return float(np.nanmin(st[:, 7])), float(np.nanmax(st[:, 9])), float(arr[:, :, 4].sum())
 
     out = {}
-    for name, kw in [("plain", {}),
-                     ("flow", dict(flow=True, flow_layers=8, flow_hidden=128, flow_train_steps=args.flow_steps,
-                                   flow_batch=256, flow_lr=1e-3, flow_warmup=args.flow_warmup,
-                                   flow_pilot=args.flow_pilot))]:
+    fk = dict(flow_layers=8, flow_hidden=128, flow_train_steps=args.flow_steps, flow_batch=256, flow_lr=1e-3,
+              flow_warmup=args.flow_warmup, flow_pilot=args.flow_pilot)
+    runs = [("plain", {}), ("flow", dict(fk, flow="auto")), ("flow_forced", dict(fk, flow="force"))]
+    for name, kw in runs:
+        if args.c4_only and name != args.c4_only:
+            continue
         t0 = time.time()
         print(f"c4 {name}: sampling", file=sys.stderr, flush=True)
         fit = m.sample(num_chains=256, num_warmup=args.warmup, num_samples=args.steps, seed=args.seed, refresh=50,
@@ -233,11 +235,16 @@ def bench_c4(args):
         if kw:
             r.update(flow_used=fit.flow_used, flow_train_seconds=fit.flow_train_seconds, flow_loss=fit.flow_loss,
                      pilot_ess_per_grad_plain=fit.pilot_ess_per_grad_plain,
-                     pilot_ess_per_grad_flow=fit.pilot_ess_per_grad_flow)
+                     pilot_ess_per_grad_flow=fit.pilot_ess_per_grad_flow,
+                     pilot_ess_per_sec_plain=fit.pilot_ess_per_sec_plain,
+                     pilot_ess_per_sec_flow=fit.pilot_ess_per_sec_flow, pilot_rhat_plain=fit.pilot_rhat_plain,
+                     pilot_rhat_flow=fit.pilot_rhat_flow)
         out[name] = r
         print(json.dumps({name: r}), file=sys.stderr, flush=True)
+    if args.c4_only:
+        return out
     return {"metric": "min bulk-ESS/sec, learned flow vs plain NUTS (centered hierarchical logistic)",
-            "value": out["flow"]["ess_per_sec_sampling"], "unit": "ESS/s", "n_gpus": 1, "steps": args.steps,
+            "value": out["flow"]["ess_per_sec_sampling"], "flow_forced": out["flow_forced"], "unit": "ESS/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "higher_is_better": True, "dtype": "f64 sampler, f32 flow", "data": "synthetic",
             "config": {"workload": "centered hierarchical logistic N=50000, 400 groups x 10 covariates (D=4020), "
                                    "256 chains, diag_e; flow 8 coupling layers, MLP 2x128 FP32"},
@@ -303,6 +310,8 @@ def main():
     ap.add_argument("--D5", type=int, default=1000, help="c5: coefficients")
     ap.add_argument("--flow-warmup", type=int, default=300, help="c4: warmup iterations on the flow")
     ap.add_argument("--flow-pilot", type=int, default=100, help="c4: pilot draws per sampler")
+    ap.add_argument("--c4-only", default="", choices=["", "plain", "flow", "flow_forced"],
+                    help="c4: run one sampler only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run a device copy of BYTES (counter calibration for profiles)")
     args = ap.parse_args()

@@ -364,7 +364,7 @@ void validate(const nuts_config& c) {
   if (c.num_chains < 1) throw std::invalid_argument("num_chains must be >= 1");
   if (c.init_radius < 0) throw std::invalid_argument("init radius must be >= 0");
   if (c.precision < 0 || c.precision > 2) throw std::invalid_argument("precision must be 0 (fp64), 1 (fp32) or 2 (bf16)");
-  if (c.flow < 0 || c.flow > 1) throw std::invalid_argument("flow must be 0 or 1");
+  if (c.flow < 0 || c.flow > 2) throw std::invalid_argument("flow must be 0, 1 or 2");
   if (c.flow) {
     if (c.flow_layers < 1 || c.flow_hidden < 1 || c.flow_train_steps < 0 || c.flow_batch < 1 || !(c.flow_lr > 0) ||
         c.flow_warmup < 1 || c.flow_pilot < 4)
@@ -925,7 +925,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
 
 // min bulk-ESS over the D unconstrained coordinates of rows [r0, r1) of every
 // chain, divided by the leapfrog steps those rows took
-double pilot_ess_per_grad(const RunResult& r, int r0, int r1, int D) {
+double pilot_ess_per_grad(const RunResult& r, int r0, int r1, int D, double* rhat_max = nullptr) {
   const int C = (int)r.st.size();
   const int n = r1 - r0;
   if (n < 4) return 0.0;
@@ -943,10 +943,14 @@ double pilot_ess_per_grad(const RunResult& r, int r0, int r1, int D) {
   std::vector<double> st((size_t)D * 10);
   if (nuts_summary(dr.data(), n, C, D, 16, st.data()) != NUTS_OK) throw std::runtime_error("pilot summary failed");
   double mn = std::numeric_limits<double>::infinity();
+  double rh = 0.0;
   for (int p = 0; p < D; ++p) {
     const double e = st[(size_t)p * 10 + 7];
     if (std::isfinite(e)) mn = std::min(mn, e);
+    const double rr = st[(size_t)p * 10 + 9];
+    if (std::isfinite(rr)) rh = std::max(rh, rr);
   }
+  if (rhat_max) *rhat_max = rh;
   if (!std::isfinite(mn) || grads <= 0) return 0.0;
   return mn / grads;
 }
@@ -958,7 +962,7 @@ bool all_done(const RunResult& r) {
 
 struct FlowReport {
   int used = 0;
-  double train_s = 0, loss = 0, egp = 0, egf = 0, pre_s = 0;
+  double train_s = 0, loss = 0, egp = 0, egf = 0, esp = 0, esf = 0, rhp = 0, rhf = 0, pre_s = 0;
   std::vector<int64_t> nlf, ngrad;  // per chain, over every phase
 };
 
@@ -983,7 +987,7 @@ RunResult run_with_flow(ModelImpl& M, const RunSpec& R0, FlowReport& rep) {
   add_counts(ra);
   if (!all_done(ra)) return ra;
   const int rowsA = cfg.num_warmup + P;
-  rep.egp = pilot_ess_per_grad(ra, cfg.num_warmup, rowsA, D);
+  rep.egp = pilot_ess_per_grad(ra, cfg.num_warmup, rowsA, D, &rep.rhp);
   // training set: second half of warmup and the pilot
   const int first = cfg.num_warmup / 2;
   const int64_t ntr = (int64_t)C * (rowsA - first);
@@ -1033,8 +1037,24 @@ RunResult run_with_flow(ModelImpl& M, const RunSpec& R0, FlowReport& rep) {
   RunResult rb = run_batch(M, Bs);
   add_counts(rb);
   const bool flow_ok = all_done(rb);
-  rep.egf = flow_ok ? pilot_ess_per_grad(rb, 0, P, D) : 0.0;
-  rep.used = flow_ok && rep.egf > rep.egp;
+  rep.egf = flow_ok ? pilot_ess_per_grad(rb, 0, P, D, &rep.rhf) : 0.0;
+  // the flow must raise min-ESS per gradient, and pay for its own cost: the
+  // pilot's min-ESS per second must rise too (gradients through the flow are
+  // dearer than plain ones)
+  auto pilot_grads = [&](const RunResult& r, int r0, int r1) {
+    double gsum = 0;
+    for (int c = 0; c < C; ++c) {
+      const double* base = r.draws.data() + r.draw_off[c];
+      for (int k = r0; k < r1; ++k) gsum += base[(int64_t)k * r.cols[c] + 4];
+    }
+    return gsum;
+  };
+  rep.esp = ra.samp_s > 0 ? rep.egp * pilot_grads(ra, cfg.num_warmup, rowsA) / ra.samp_s : 0.0;
+  rep.esf = flow_ok && rb.samp_s > 0 ? rep.egf * pilot_grads(rb, 0, P) / rb.samp_s : 0.0;
+  // ... unless plain NUTS is visibly not mixing (pilot R-hat above 1.05) and the
+  // flow mixes better: then the speed of unconverged plain draws is no argument
+  const bool plain_stuck = rep.rhp > 1.05 && rep.rhf < rep.rhp;
+  rep.used = flow_ok && (cfg.flow == 2 || (rep.egf > rep.egp && (rep.esf > rep.esp || plain_stuck)));
   rep.pre_s = ra.warm_s + ra.samp_s + rep.train_s + rb.warm_s + rb.samp_s;
   // phase C: the better sampler continues without adaptation
   const RunResult& src = rep.used ? rb : ra;
@@ -1319,6 +1339,10 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
         out->flow_loss = rep.loss;
         out->pilot_ess_per_grad_plain = rep.egp;
         out->pilot_ess_per_grad_flow = rep.egf;
+        out->pilot_ess_per_sec_plain = rep.esp;
+        out->pilot_ess_per_sec_flow = rep.esf;
+        out->pilot_rhat_plain = rep.rhp;
+        out->pilot_rhat_flow = rep.rhf;
       }
     }
     for (int c = 0; c < C; ++c)

@@ -138,7 +138,8 @@ class Model:
         cfg.num_chains, cfg.id, cfg.seed, cfg.fused = num_chains, id, seed, fused
         cfg.precision = {"fp64": 0, "fp32": 1, "bf16": 2}.get(precision, precision)
         cfg.max_saved_params = max_saved_params
-        cfg.flow, cfg.flow_layers, cfg.flow_hidden = int(flow), flow_layers, flow_hidden
+        cfg.flow = {"off": 0, "auto": 1, "force": 2}.get(flow, int(flow) if not isinstance(flow, str) else -1)
+        cfg.flow_layers, cfg.flow_hidden = flow_layers, flow_hidden
         cfg.flow_train_steps, cfg.flow_batch, cfg.flow_lr = flow_train_steps, flow_batch, flow_lr
         cfg.flow_warmup, cfg.flow_pilot = flow_warmup, flow_pilot
         cfg.refresh = refresh
@@ -198,7 +199,10 @@ class Model:
                   grad_chains_per_launch=out.grad_chains_per_launch, flow_used=bool(out.flow_used),
                   flow_train_seconds=out.flow_train_seconds, flow_loss=out.flow_loss,
                   pilot_ess_per_grad_plain=out.pilot_ess_per_grad_plain,
-                  pilot_ess_per_grad_flow=out.pilot_ess_per_grad_flow, tuned_path=out.tuned_path,
+                  pilot_ess_per_grad_flow=out.pilot_ess_per_grad_flow,
+                  pilot_ess_per_sec_plain=out.pilot_ess_per_sec_plain,
+                  pilot_ess_per_sec_flow=out.pilot_ess_per_sec_flow, pilot_rhat_plain=out.pilot_rhat_plain,
+                  pilot_rhat_flow=out.pilot_rhat_flow, tuned_path=out.tuned_path,
                   tuned_blocks_per_cu=out.tuned_blocks_per_cu, column_names=names,
                   dataset_of_chain=ds_of_chain, config=cfg, error=err)
         if rc != 0:

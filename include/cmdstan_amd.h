@@ -111,10 +111,12 @@ typedef struct nuts_config {
    * adaptive warmup plus flow_pilot draws, trains an affine-coupling flow on the
    * second half of warmup and the pilot draws, runs flow_warmup adaptive
    * iterations and flow_pilot draws of NUTS on the pulled-back density, and then
-   * samples num_samples iterations with whichever of the two gave the larger
-   * min bulk-ESS per gradient evaluation over the pilot (plain NUTS is the
-   * fallback).  One dataset; at least 2 dimensions; lockstep kernels. */
-  int flow;               /* 0 (default) off, 1 on */
+   * samples num_samples iterations with the flow only if its pilot raised min
+   * bulk-ESS per gradient evaluation over the plain pilot and either raised
+   * min bulk-ESS per second too or mixed where plain NUTS did not (pilot max
+   * R-hat above 1.05 for plain NUTS and lower for the flow); plain NUTS is the
+   * fallback.  One dataset; at least 2 dimensions; lockstep kernels. */
+  int flow;               /* 0 (default) off, 1 on with the plain-NUTS fallback, 2 on without it */
   int flow_layers;        /* 8 coupling layers */
   int flow_hidden;        /* 128: width of the two tanh hidden layers */
   int flow_train_steps;   /* 1000 Adam steps */
@@ -167,6 +169,10 @@ typedef struct nuts_output {
   double flow_loss;               /* mean negative log-likelihood of the last training batches */
   double pilot_ess_per_grad_plain;
   double pilot_ess_per_grad_flow;
+  double pilot_ess_per_sec_plain; /* the flow is used only if it raises both, or */
+  double pilot_ess_per_sec_flow;
+  double pilot_rhat_plain;        /* if plain NUTS does not mix (max R-hat > 1.05) */
+  double pilot_rhat_flow;         /* and the flow mixes better */
   /* batched GLM gradient chosen by the autotuner (lockstep logistic regression):
      path 0 single-pass kernel with tuned_blocks_per_cu blocks per CU, 1 two-pass
      kernels; -1 when not applicable */

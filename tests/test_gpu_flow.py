@@ -89,18 +89,25 @@ def test_adam_training_fits_correlated_gaussian():
 
 def test_flow_sampler_on_correlated_gaussian_gains_and_stays_exact():
     """NUTS on the pulled-back density of a correlated gaussian (diag_e cannot
-    absorb the correlation): the learned flow is chosen, raises ESS per
-    gradient over plain NUTS, and the draws keep the target's moments."""
+    absorb the correlation): the trained flow raises ESS per gradient over
+    plain NUTS, the automatic choice follows its rule, and NUTS through the
+    flow keeps the target's moments."""
     D = 8
     rho = 0.95
     Sig = rho * np.ones((D, D)) + (1 - rho) * np.eye(D)
     m = ca.Model("gauss", dict(D=D, P=np.linalg.inv(Sig)))
-    fit = m.sample(num_chains=64, num_warmup=400, num_samples=300, seed=11, flow=True, flow_layers=4,
-                   flow_hidden=32, flow_train_steps=800, flow_batch=256, flow_lr=3e-3, flow_warmup=200,
-                   flow_pilot=60)
-    assert np.all(fit.status == 0)
-    assert fit.flow_used, (fit.pilot_ess_per_grad_plain, fit.pilot_ess_per_grad_flow)
-    assert fit.pilot_ess_per_grad_flow > 1.5 * fit.pilot_ess_per_grad_plain
+    kw = dict(num_chains=64, num_warmup=400, num_samples=300, seed=11, flow_layers=4, flow_hidden=32,
+              flow_train_steps=800, flow_batch=256, flow_lr=3e-3, flow_warmup=200, flow_pilot=60)
+    auto = m.sample(flow="auto", **kw)
+    assert np.all(auto.status == 0)
+    # the trained network raises ESS per gradient ...
+    assert auto.pilot_ess_per_grad_flow > 1.5 * auto.pilot_ess_per_grad_plain
+    # ... and the sampler keeps it only if ESS per second rises as well
+    stuck = auto.pilot_rhat_plain > 1.05 and auto.pilot_rhat_flow < auto.pilot_rhat_plain
+    assert auto.flow_used == (auto.pilot_ess_per_grad_flow > auto.pilot_ess_per_grad_plain and
+                              (auto.pilot_ess_per_sec_flow > auto.pilot_ess_per_sec_plain or stuck))
+    fit = m.sample(flow="force", **kw)
+    assert np.all(fit.status == 0) and fit.flow_used
     x = np.stack([fit.param(f"x.{i + 1}") for i in range(D)])
     for i in range(D):
         s = E.summary(x[i])
@@ -115,14 +122,14 @@ def test_flow_sampler_on_correlated_gaussian_gains_and_stays_exact():
 
 
 def test_flow_sampler_eight_schools_matches_plain_moments():
-    """Centered eight schools (funnel): whichever sampler the pilot picks, the
-    draws agree with a long plain FP64 run within 4 combined MCSE."""
+    """Centered eight schools (funnel): NUTS through the trained flow agrees
+    with a long plain FP64 run within 4 combined MCSE."""
     data = dict(J=8, y=[28, 8, -3, 7, -1, 1, 18, 12], sigma=[15, 10, 16, 11, 9, 11, 10, 18])
     m = ca.Model("eight_schools_cp", data)
     ref = m.sample(num_chains=64, num_warmup=1000, num_samples=1000, seed=3, delta=0.95)
-    fit = m.sample(num_chains=64, num_warmup=600, num_samples=600, seed=4, delta=0.95, flow=True, flow_layers=4,
+    fit = m.sample(num_chains=64, num_warmup=600, num_samples=600, seed=4, delta=0.95, flow="force", flow_layers=4,
                    flow_hidden=32, flow_train_steps=600, flow_pilot=80, flow_warmup=300)
-    assert np.all(fit.status == 0)
+    assert np.all(fit.status == 0) and fit.flow_used
     for name in ["mu", "tau", "theta.1"]:
         a, b = E.summary(ref.param(name)), E.summary(fit.param(name))
         assert abs(a["mean"] - b["mean"]) < 4 * np.hypot(a["mcse"], b["mcse"]), (name, a, b)

@@ -389,7 +389,7 @@ def test_graph_replay_matches_eager_launches(family, monkeypatch):
 def test_graph_replay_matches_eager_with_flow(monkeypatch):
     from tests.model_cases import gauss
     m = ca.Model("gauss", gauss(seed=6, D=6))
-    kw = dict(num_chains=32, num_warmup=150, num_samples=60, seed=2, flow=True, flow_layers=2, flow_hidden=16,
+    kw = dict(num_chains=32, num_warmup=150, num_samples=60, seed=2, flow="force", flow_layers=2, flow_hidden=16,
               flow_train_steps=100, flow_warmup=60, flow_pilot=20)
     monkeypatch.setenv("CMDSTAN_AMD_GRAPH", "0")
     a = m.sample(**kw)

@@ -145,8 +145,9 @@ def bench_c3(args):
     datasets = make_c3(nm, args.seed)
     keep = 10  # mu, log tau, eta[1..8] per chain
     m = ca.Model("hier_normal_nc", datasets)
+    print(f"c3: {nm} models, batched", file=sys.stderr, flush=True)
     fit = m.sample(num_chains=64, num_warmup=args.warmup, num_samples=args.steps, seed=args.seed,
-                   max_saved_params=keep)
+                   max_saved_params=keep, refresh=50)
     arr = np.stack(fit.draws)  # chains x draws x (7 + keep)
     C = arr.shape[0]
     ess = []
@@ -161,9 +162,10 @@ def bench_c3(args):
     picks = np.argsort(Js)[np.linspace(0, nm - 1, min(8, nm)).astype(int)]
     t_plain = []
     for k in picks:
+        print(f"c3: model J={Js[k]} alone", file=sys.stderr, flush=True)
         f1 = ca.Model("hier_normal_nc", datasets[k]).sample(num_chains=64, num_warmup=args.warmup,
                                                              num_samples=args.steps, seed=args.seed,
-                                                             max_saved_params=keep)
+                                                             max_saved_params=keep, refresh=100)
         t_plain.append(f1.sampling_seconds)
     # per-model time grows with J: fit time = a + b J over the picks
     A = np.column_stack([np.ones(len(picks)), Js[picks]])

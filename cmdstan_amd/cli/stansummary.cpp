@@ -1,0 +1,266 @@
+// stansummary over Stan CSV files written by the sampler (one file per chain):
+// the statistics of stansummary_helper.hpp:397-428 (get_stats) through
+// nuts_summary, the console table and the --csv_filename output
+// (src/cmdstan/stansummary.cpp:29-120 options).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cmdstan_amd.h"
+
+namespace {
+
+struct ChainFile {
+  std::vector<std::string> names;
+  std::vector<std::vector<double>> rows;
+  std::map<std::string, std::string> meta;  // "key = value" comment lines
+  double warmup_s = 0, sampling_s = 0;
+};
+
+std::string trim(const std::string& s) {
+  const size_t a = s.find_first_not_of(" \t\r\n");
+  if (a == std::string::npos) return "";
+  const size_t b = s.find_last_not_of(" \t\r\n");
+  return s.substr(a, b - a + 1);
+}
+
+std::vector<std::string> split(const std::string& s, char sep) {
+  std::vector<std::string> out;
+  std::string cur;
+  std::istringstream is(s);
+  while (std::getline(is, cur, sep)) out.push_back(cur);
+  return out;
+}
+
+ChainFile read_csv(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("Can't open file " + path);
+  ChainFile f;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.empty()) continue;
+    if (line[0] == '#') {
+      const std::string body = trim(line.substr(1));
+      const size_t eq = body.find('=');
+      if (eq != std::string::npos) {
+        std::string k = trim(body.substr(0, eq)), v = trim(body.substr(eq + 1));
+        const size_t sp = v.find(' ');
+        if (sp != std::string::npos && v.find("(Default)") != std::string::npos) v = v.substr(0, sp);
+        if (!f.meta.count(k)) f.meta[k] = v;
+      }
+      // "#  Elapsed Time: 0.005 seconds (Warm-up)"
+      const size_t et = body.find("Elapsed Time:");
+      const std::string tb = et != std::string::npos ? body.substr(et + 13) : body;
+      if (tb.find("(Warm-up)") != std::string::npos) f.warmup_s = std::atof(trim(tb).c_str());
+      else if (tb.find("(Sampling)") != std::string::npos) f.sampling_s = std::atof(trim(tb).c_str());
+      continue;
+    }
+    if (f.names.empty()) {
+      f.names = split(trim(line), ',');
+      continue;
+    }
+    const auto cells = split(trim(line), ',');
+    if (cells.size() != f.names.size()) continue;
+    std::vector<double> r(cells.size());
+    for (size_t i = 0; i < cells.size(); ++i) r[i] = std::strtod(cells[i].c_str(), nullptr);
+    f.rows.push_back(std::move(r));
+  }
+  if (f.names.empty()) throw std::runtime_error("no header in " + path);
+  return f;
+}
+
+double quantile7(std::vector<double> v, double p) {
+  std::sort(v.begin(), v.end());
+  const double h = (v.size() - 1) * p;
+  const size_t lo = (size_t)std::floor(h);
+  const size_t hi = std::min(lo + 1, v.size() - 1);
+  return v[lo] + (h - lo) * (v[hi] - v[lo]);
+}
+
+// a.1.2 -> a[1,2]
+std::string display_name(const std::string& n) {
+  const size_t dot = n.find('.');
+  if (dot == std::string::npos) return n;
+  std::string r = n.substr(0, dot) + "[";
+  std::string rest = n.substr(dot + 1);
+  for (char& ch : rest) if (ch == '.') ch = ',';
+  return r + rest + "]";
+}
+
+std::string fmt_sig(double v, int sig) {
+  std::ostringstream os;
+  if (!std::isfinite(v)) {
+    os << v;
+    return os.str();
+  }
+  const double a = std::fabs(v);
+  if (a != 0 && (a >= 1e6 || a < 1e-3)) {
+    os << std::scientific << std::setprecision(std::max(0, sig - 1)) << v;
+  } else {
+    const int ip = a >= 1 ? (int)std::floor(std::log10(a)) + 1 : 0;
+    const int prec = std::max(0, sig - ip);
+    os << std::fixed << std::setprecision(ip >= sig ? 0 : prec) << v;
+  }
+  return os.str();
+}
+
+int usage() {
+  std::cout << "Usage: stansummary [OPTIONS] stan_csv_file(s)\n"
+               "  -c, --csv_filename [file]   Write statistics to a csv file.\n"
+               "  -p, --percentiles [values]  Percentiles to report, default \"5,50,95\".\n"
+               "  -s, --sig_figs [n]          Significant figures reported. Default is 2.\n"
+               "  -i, --include_param [name]  Include the named parameter in the summary output.\n";
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::vector<std::string> files, include;
+  std::string csv_filename, pct_spec = "5,50,95";
+  bool have_csv = false;
+  int sig_figs = 2;
+  try {
+    for (int i = 1; i < argc; ++i) {
+      std::string a = argv[i];
+      auto value = [&](const std::string& longname, const std::string& shortname) -> std::string {
+        if (a.rfind(longname + "=", 0) == 0) return a.substr(longname.size() + 1);
+        if (a == longname || a == shortname) {
+          if (i + 1 >= argc) throw std::invalid_argument(longname + ": requires a value");
+          return argv[++i];
+        }
+        return std::string("\x01");
+      };
+      if (a == "-h" || a == "--help") return usage();
+      std::string v;
+      if ((v = value("--csv_filename", "-c")) != "\x01") { csv_filename = v; have_csv = true; }
+      else if ((v = value("--percentiles", "-p")) != "\x01") pct_spec = v;
+      else if ((v = value("--sig_figs", "-s")) != "\x01") {
+        sig_figs = std::atoi(v.c_str());
+        if (sig_figs < 1 || sig_figs > 18) {
+          std::cout << "--sig_figs: Value " << v << " not in range 1 to 18" << std::endl;
+          return 1;
+        }
+      } else if ((v = value("--include_param", "-i")) != "\x01") include.push_back(v);
+      else if (!a.empty() && a[0] == '-') throw std::invalid_argument("unknown option " + a);
+      else files.push_back(a);
+    }
+    if (files.empty()) return usage(), 1;
+    std::vector<std::string> pcts;
+    for (auto& p : split(pct_spec, ',')) {
+      const std::string t = trim(p);
+      if (t.empty() || t == "\"\"") continue;
+      const double x = std::atof(t.c_str());
+      if (!(x >= 0 && x <= 100)) throw std::invalid_argument("Option --percentiles " + pct_spec + ": values must be in (0,100)");
+      pcts.push_back(t);
+    }
+    std::vector<ChainFile> chains;
+    for (auto& f : files) chains.push_back(read_csv(f));
+    const auto& names = chains[0].names;
+    size_t n = chains[0].rows.size();
+    for (auto& c : chains) {
+      if (c.names != names) throw std::runtime_error("the files have different columns");
+      n = std::min(n, c.rows.size());
+    }
+    if (n < 2) throw std::runtime_error("need at least 2 draws per chain");
+    // column selection: sampler columns, then requested (or all) model columns
+    std::vector<int> cols;
+    for (size_t j = 0; j < names.size(); ++j) {
+      const bool sampler = names[j].size() > 2 && names[j].compare(names[j].size() - 2, 2, "__") == 0;
+      bool keep = include.empty() || sampler;
+      for (auto& inc : include) {
+        const std::string dn = display_name(names[j]);
+        if (names[j] == inc || dn == inc || dn.rfind(inc + "[", 0) == 0) keep = true;
+      }
+      if (keep) cols.push_back((int)j);
+    }
+    const int P = (int)cols.size(), M = (int)chains.size();
+    std::vector<double> dr((size_t)P * M * n);
+    for (int p = 0; p < P; ++p)
+      for (int c = 0; c < M; ++c)
+        for (size_t k = 0; k < n; ++k) dr[((size_t)p * M + c) * n + k] = chains[c].rows[k][cols[p]];
+    std::vector<double> st((size_t)P * 10);
+    if (nuts_summary(dr.data(), (int)n, M, P, 8, st.data()) != NUTS_OK) throw std::runtime_error(nuts_last_error());
+    // rows: Mean MCSE StdDev MAD pcts... ESS_bulk ESS_tail R_hat
+    std::vector<std::string> header = {"Mean", "MCSE", "StdDev", "MAD"};
+    for (auto& p : pcts) header.push_back(p + "%");
+    header.insert(header.end(), {"ESS_bulk", "ESS_tail", "R_hat"});
+    std::vector<std::vector<double>> table(P);
+    for (int p = 0; p < P; ++p) {
+      const double* s = &st[(size_t)p * 10];
+      std::vector<double> all(dr.begin() + (size_t)p * M * n, dr.begin() + (size_t)(p + 1) * M * n);
+      std::vector<double> row = {s[0], s[1], s[2], s[3]};
+      for (auto& q : pcts) {
+        const double pr = std::atof(q.c_str()) / 100.0;
+        row.push_back(pr == 0.05 ? s[4] : pr == 0.5 ? s[5] : pr == 0.95 ? s[6] : quantile7(all, pr));
+      }
+      row.insert(row.end(), {s[7], s[8], s[9]});
+      table[p] = row;
+    }
+    // console
+    const ChainFile& c0 = chains[0];
+    auto meta = [&](const std::string& k, const std::string& d) { return c0.meta.count(k) ? c0.meta.at(k) : d; };
+    std::cout << "Inference for Stan model: " << meta("model", "model") << "\n"
+              << M << " chains: each with iter=" << meta("num_samples", std::to_string(n))
+              << "; warmup=" << meta("num_warmup", "0") << "; thin=" << meta("thin", "1") << "; " << n * M
+              << " iterations saved.\n\n";
+    double tw = 0, ts = 0;
+    for (auto& c : chains) { tw += c.warmup_s; ts += c.sampling_s; }
+    std::cout << std::fixed << std::setprecision(3) << "Warmup took " << tw << " seconds\nSampling took " << ts
+              << " seconds\n\n" << std::defaultfloat;
+    size_t wname = 4;
+    for (int p = 0; p < P; ++p) wname = std::max(wname, display_name(names[cols[p]]).size());
+    std::vector<std::vector<std::string>> cells(P, std::vector<std::string>(header.size()));
+    std::vector<size_t> w(header.size());
+    for (size_t h = 0; h < header.size(); ++h) w[h] = header[h].size();
+    for (int p = 0; p < P; ++p)
+      for (size_t h = 0; h < header.size(); ++h) {
+        const bool ess = header[h] == "ESS_bulk" || header[h] == "ESS_tail";
+        cells[p][h] = ess ? (std::isfinite(table[p][h]) ? std::to_string((long long)std::llround(table[p][h])) : "nan")
+                          : fmt_sig(table[p][h], sig_figs);
+        w[h] = std::max(w[h], cells[p][h].size());
+      }
+    std::cout << std::string(wname, ' ');
+    for (size_t h = 0; h < header.size(); ++h) std::cout << "  " << std::setw((int)w[h]) << header[h];
+    std::cout << "\n\n";
+    for (int p = 0; p < P; ++p) {
+      std::cout << std::left << std::setw((int)wname) << display_name(names[cols[p]]) << std::right;
+      for (size_t h = 0; h < header.size(); ++h) std::cout << "  " << std::setw((int)w[h]) << cells[p][h];
+      std::cout << "\n";
+      if (names[cols[p]] == "energy__") std::cout << "\n";
+    }
+    std::cout << "\nSamples were drawn using " << meta("algorithm", "hmc") << " with " << meta("engine", "nuts")
+              << ".\nFor each parameter, ESS_bulk and ESS_tail measure the effective sample size for the entire "
+                 "sample (bulk) and for the .05 and .95 tails (tail), \nand R_hat measures the potential scale "
+                 "reduction on split chains. At convergence R_hat will be very close to 1.00.\n";
+    if (have_csv) {
+      std::ofstream out(csv_filename);
+      if (!out) {
+        std::cout << "Cannot save to csv_filename: " << csv_filename << "." << std::endl;
+        return 1;
+      }
+      out << "name";
+      for (auto& h : header) out << "," << h;
+      out << "\n";
+      out << std::setprecision(6);
+      for (int p = 0; p < P; ++p) {
+        out << "\"" << display_name(names[cols[p]]) << "\"";
+        for (double v : table[p]) out << "," << v;
+        out << "\n";
+      }
+    }
+  } catch (const std::exception& e) {
+    std::cout << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,43 @@
+"""The stansummary CLI on the reference's example output: its --csv_filename
+rows equal the golden strings of stansummary_test.cpp:481-550 (CPU only)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "cmdstan_amd", "bin", "stansummary")
+CSV = os.path.join(ROOT, "tests", "golden", "bernoulli_chain_1.csv")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(BIN), reason="CLI not built")
+
+
+def _run(args, tmp_path):
+    out = tmp_path / "summary.csv"
+    r = subprocess.run([BIN, f"--csv_filename={out}"] + args + [CSV], capture_output=True, text=True, timeout=60)
+    return r, (out.read_text().splitlines() if out.exists() else [])
+
+
+def test_csv_output_matches_golden_rows(tmp_path):
+    r, lines = _run([], tmp_path)
+    assert r.returncode == 0, r.stdout
+    assert lines[0] == "name,Mean,MCSE,StdDev,MAD,5%,50%,95%,ESS_bulk,ESS_tail,R_hat"
+    assert lines[1] == '"lp__",-7.2719,0.0365168,0.768874,0.303688,-8.98426,-6.97009,-6.75007,519.29,503.309,1.00141'
+    assert lines[7] == ('"energy__",7.78428,0.0508815,1.0314,0.745859,6.80565,7.46758,9.8864,489.874,486.438,'
+                        '1.00495')
+    assert lines[8] == ('"theta",0.256552,0.00610844,0.119654,0.120965,0.0802982,0.24996,0.47034,361.506,395.736,'
+                        '1.00186')
+    assert "Inference for Stan model: bernoulli_model" in r.stdout
+
+
+def test_csv_output_without_percentiles(tmp_path):
+    r, lines = _run(["-p", ""], tmp_path)
+    assert r.returncode == 0
+    assert lines[0] == "name,Mean,MCSE,StdDev,MAD,ESS_bulk,ESS_tail,R_hat"
+    assert lines[1] == '"lp__",-7.2719,0.0365168,0.768874,0.303688,519.29,503.309,1.00141'
+
+
+def test_bad_sig_figs_is_an_error(tmp_path):
+    r = subprocess.run([BIN, "--sig_figs", "-1", CSV], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "--sig_figs: Value -1 not in range" in r.stdout

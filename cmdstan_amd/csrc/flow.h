@@ -92,13 +92,11 @@ class FlowNet {
   float* params_ = nullptr; float* grad_ = nullptr; float* adam_m_ = nullptr; float* adam_v_ = nullptr;
   double* loc_ = nullptr; double* scale_ = nullptr; double log_scale_sum_ = 0;
   float* U_ = nullptr;     // (L+1) x R x D layer inputs/outputs
-  float* UA_ = nullptr;    // R x Dmax_a
   float* H1_ = nullptr; float* H2_ = nullptr;  // L x R x H
   float* O_ = nullptr;     // R x 2 Dmax_b
   float* S_ = nullptr;     // L x R x Dmax_b
   float* GO_ = nullptr;    // R x 2 Dmax_b
   float* GP2_ = nullptr; float* GP1_ = nullptr;  // R x H
-  float* GUA_ = nullptr;   // R x Dmax_a
   float* GC_ = nullptr;    // R x D
   float* part_ = nullptr; int64_t part_n_ = 0;  // split-K partials
   double* rowbuf_ = nullptr; double* rowbuf2_ = nullptr;  // R

@@ -146,15 +146,6 @@ __global__ void flow_gather_z(const double* __restrict__ Q, int64_t ldq, const i
   U0[i] = (float)Q[c * ldq + j];
 }
 
-// UA[r][j] = U[r][ia[j]]
-__global__ void flow_gather_cols(const float* __restrict__ U, int D, const int* __restrict__ ia, int Da,
-                                 const int* __restrict__ n_dev, int n, float* __restrict__ UA) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = (int)(i / Da), j = (int)(i % Da);
-  if (r >= rows_of(n_dev, n)) return;
-  UA[i] = U[(int64_t)r * D + ia[j]];
-}
-
 // forward coupling: Un = U with b-part u_b e^s + t; S[r][j] = s
 __global__ void flow_couple_fwd(const float* __restrict__ U, const float* __restrict__ O, const int* __restrict__ ia,
                                 const int* __restrict__ ib, int Da, int Db, int D, const int* __restrict__ n_dev, int n,
@@ -279,15 +270,6 @@ __global__ void flow_couple_bwd_train(float* __restrict__ GC, const float* __res
   GO[(int64_t)r * 2 * Db + j] = (inv_b - gu * Uz[e]) * (1.0f - q * q);
   GO[(int64_t)r * 2 * Db + Db + j] = -gu * ems;
   GC[e] = gu * ems;
-}
-
-// GC[r][ia[j]] += GUA[r][j]
-__global__ void flow_scatter_add(float* __restrict__ GC, const float* __restrict__ GUA, const int* __restrict__ ia,
-                                 int Da, int D, const int* __restrict__ n_dev, int n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = (int)(i / Da), j = (int)(i % Da);
-  if (r >= rows_of(n_dev, n)) return;
-  GC[(int64_t)r * D + ia[j]] += GUA[i];
 }
 
 // G[idx[r]] = GC[r]; V[idx[r]] = VX[r] - logdet[r]
@@ -456,13 +438,11 @@ FlowNet::FlowNet(const FlowSpec& spec, int max_rows) : spec_(spec), R_(max_rows)
   loc_ = dalloc<double>(D); scale_ = dalloc<double>(D);
   const int64_t R = R_;
   U_ = dalloc<float>((L + 1) * R * D);
-  UA_ = dalloc<float>(R * Dmax_a_);
   H1_ = dalloc<float>(L * R * H); H2_ = dalloc<float>(L * R * H);
   O_ = dalloc<float>(R * 2 * Dmax_b_);
   S_ = dalloc<float>(L * R * Dmax_b_);
   GO_ = dalloc<float>(R * 2 * Dmax_b_);
   GP2_ = dalloc<float>(R * H); GP1_ = dalloc<float>(R * H);
-  GUA_ = dalloc<float>(R * Dmax_a_);
   GC_ = dalloc<float>(R * D);
   rowbuf_ = dalloc<double>(R);
   rowbuf2_ = dalloc<double>(R);
@@ -487,8 +467,8 @@ FlowNet::~FlowNet() {
   }
   for (auto& l : layers_) { (void)hipFree(l.ia); (void)hipFree(l.ib); }
   for (void* p : {(void*)params_, (void*)grad_, (void*)adam_m_, (void*)adam_v_, (void*)loc_, (void*)scale_, (void*)U_,
-                  (void*)UA_, (void*)H1_, (void*)H2_, (void*)O_, (void*)S_, (void*)GO_, (void*)GP2_, (void*)GP1_,
-                  (void*)GUA_, (void*)GC_, (void*)part_, (void*)rowbuf_, (void*)rowbuf2_, (void*)pick_, (void*)xrows_,
+                  (void*)H1_, (void*)H2_, (void*)O_, (void*)S_, (void*)GO_, (void*)GP2_, (void*)GP1_,
+                  (void*)GC_, (void*)part_, (void*)rowbuf_, (void*)rowbuf2_, (void*)pick_, (void*)xrows_,
                   (void*)db_dev_})
     if (p) (void)hipFree(p);
 }

@@ -181,9 +181,12 @@ struct NutsWave {
       double acc[RB];
 #pragma unroll
       for (int r = 0; r < RB; ++r) acc[r] = 0.0;
+      const double* __restrict__ xr = x;
+      const double* __restrict__ Mr = dM;
+#pragma unroll 4
       for (int j = 0; j < D; ++j) {
-        const double xj = x[j];
-        const double* col = dM + (int64_t)j * D;  // column j = row j of Minv
+        const double xj = xr[j];
+        const double* col = Mr + (int64_t)j * D;  // column j = row j of Minv
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
           const int i = i0 + tid + TPC * r;

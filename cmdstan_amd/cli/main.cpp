@@ -298,8 +298,8 @@ int run(int argc, char** argv) {
   const std::string init = val(R, {"init"});
   char* ie = nullptr;
   const double radius = std::strtod(init.c_str(), &ie);
-  if (init.empty() || *ie) throw std::invalid_argument("init files are not supported by this build; give a radius");
-  cfg.init_radius = radius;
+  const bool init_is_file = init.empty() || *ie;
+  if (!init_is_file) cfg.init_radius = radius;
   const long long refresh = to_int(val(R, {"output", "refresh"}), "refresh");
   if (refresh < 0) throw std::invalid_argument("refresh must be >= 0");
   cfg.refresh = (int)refresh;
@@ -360,7 +360,33 @@ int run(int argc, char** argv) {
   out.draws = draws.data(); out.stepsize = step.data(); out.inv_metric = imet.data();
   out.n_leapfrog = nlf.data(); out.grad_evals = nge.data(); out.status = status.data();
   std::cout << "Running " << C << " chain(s) of NUTS on the GPU (" << nuts_version() << ")" << std::endl;
-  const int rc = nuts_sample(model, &cfg, nullptr, inv_metric.empty() ? nullptr : inv_metric.data(), &out);
+  // init=<file>: per-chain files {name}_{id+i}{ext} when the first exists, else
+  // the named file for every chain (command_helper.hpp:144-245)
+  std::vector<double> inits;
+  if (init_is_file) {
+    const size_t dot = init.find_last_of('.');
+    if (dot == std::string::npos || init.substr(dot) != ".json")
+      throw std::invalid_argument("Found: \"" + init + "\" but user specified files must end in .json");
+    std::vector<std::string> files(cfg.num_chains, init);
+    if (cfg.num_chains > 1) {
+      const auto per = make_filenames(init.substr(0, dot), "", ".json", cfg.num_chains, cfg.id);
+      if (std::ifstream(per[0]).good()) files = per;
+      else if (!std::ifstream(init).good())
+        throw std::invalid_argument("Can't open either of specified files, \"" + per[0] + "\" and base file \"" +
+                                    init + "\"");
+    }
+    inits.resize((size_t)cfg.num_chains * D);
+    for (int c = 0; c < cfg.num_chains; ++c) {
+      if (!std::ifstream(files[c]).good()) throw std::invalid_argument("Can't open init file " + files[c]);
+      nuts_data* idata = nuts_data_create();
+      const bool ok = nuts_data_parse_json(idata, read_file(files[c]).c_str()) == NUTS_OK &&
+                      nuts_model_transform_inits(model, 0, idata, inits.data() + (size_t)c * D) == NUTS_OK;
+      nuts_data_destroy(idata);
+      if (!ok) throw std::invalid_argument(nuts_last_error());
+    }
+  }
+  const int rc = nuts_sample(model, &cfg, inits.empty() ? nullptr : inits.data(),
+                             inv_metric.empty() ? nullptr : inv_metric.data(), &out);
   if (rc != NUTS_OK) {
     std::cerr << nuts_last_error() << std::endl;
     for (int k = 0; k < C; ++k)

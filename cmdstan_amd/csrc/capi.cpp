@@ -1268,6 +1268,57 @@ static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n
   });
 }
 
+// stan::model::model_base::transform_inits for the registered families: the
+// named constrained values of `inits` to the unconstrained vector (D values);
+// parameters absent from `inits` are NaN (drawn at random by initialize).
+int nuts_model_transform_inits(nuts_model* m, int dataset, const nuts_data* inits, double* out) {
+  return guarded([&]() {
+    ModelImpl& M = m->impl;
+    if (dataset < 0 || dataset >= (int)M.ds.size()) throw std::out_of_range("dataset index");
+    const Dataset& ds = M.ds[dataset];
+    const int D = ds.D;
+    for (int i = 0; i < D; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+    auto take = [&](const std::string& name, int off, int n, int kind) {
+      // kind 0 identity, 1 lower=0 (log), 2 (0,1) (logit)
+      if (!inits->has(name)) return;
+      const Field& f = inits->get(name);
+      if ((int)f.v.size() != n)
+        throw std::invalid_argument("init: " + name + " has " + std::to_string(f.v.size()) + " values, expected " +
+                                    std::to_string(n));
+      for (int i = 0; i < n; ++i) {
+        const double v = f.v[i];
+        if (!std::isfinite(v)) throw std::invalid_argument("init: " + name + " must be finite");
+        if (kind == 1) {
+          if (!(v > 0)) throw std::domain_error("init: " + name + " is " + std::to_string(v) + ", but must be > 0");
+          out[off + i] = std::log(v);
+        } else if (kind == 2) {
+          if (!(v > 0 && v < 1))
+            throw std::domain_error("init: " + name + " is " + std::to_string(v) + ", but must be in (0, 1)");
+          out[off + i] = std::log(v / (1 - v));
+        } else {
+          out[off + i] = v;
+        }
+      }
+    };
+    switch (M.kind) {
+      case MK_BERNOULLI: take("theta", 0, 1, 2); break;
+      case MK_GAUSS: take("x", 0, D, 0); break;
+      case MK_LOGISTIC: take("beta", 0, D, 0); break;
+      case MK_EIGHT_SCHOOLS: take("mu", 0, 1, 0); take("tau", 1, 1, 1); take("theta", 2, D - 2, 0); break;
+      case MK_HIER_NORMAL_NC: take("mu", 0, 1, 0); take("tau", 1, 1, 1); take("eta", 2, D - 2, 0); break;
+      case MK_HIER_LOGISTIC: {
+        const int K = ds.group_size;
+        take("mu", 0, K, 0);
+        take("tau", K, K, 1);
+        take("beta", 2 * K, D - 2 * K, 0);  // beta[G][K], row-major
+        break;
+      }
+      default: throw std::invalid_argument("init: unknown model family");
+    }
+    return NUTS_OK;
+  });
+}
+
 int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points, int jacobian, double* lp,
                        double* grad) {
   return log_prob_grad_impl(m, dataset, q, n_points, jacobian, 0, lp, grad);

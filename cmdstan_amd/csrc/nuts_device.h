@@ -339,9 +339,13 @@ struct NutsWave {
     const int t = s.init_tries;
     for (int i = tid; i < D; i += TPC) {
       double v;
-      if (B.user_init) v = B.user_init[B.off[c] + i];
-      else if (P.init_radius == 0) v = 0.0;
-      else v = P.init_radius * (2.0 * stream_uniform(P.seed, uid, STREAM_INIT, (uint64_t)t * (uint64_t)D + i) - 1.0);
+      // user values where given; NaN (a parameter the init file leaves out)
+      // and no user init at all draw uniform(-R, R) on the unconstrained scale
+      v = B.user_init ? B.user_init[B.off[c] + i] : __builtin_nan("");
+      if (isnan(v)) {
+        if (P.init_radius == 0) v = 0.0;
+        else v = P.init_radius * (2.0 * stream_uniform(P.seed, uid, STREAM_INIT, (uint64_t)t * (uint64_t)D + i) - 1.0);
+      }
       q[i] = v;
     } converge();
     return request(PH_INIT_CHECK);

@@ -106,6 +106,15 @@ class Model:
     def dim(self):
         return self.dims[0]
 
+    def transform_inits(self, inits, dataset=0):
+        """Constrained values by parameter name (dict or JSON text) to the
+        unconstrained vector; parameters left out are NaN (drawn at random)."""
+        d = inits if isinstance(inits, Data) else (Data(json_text=inits) if isinstance(inits, str) else Data(inits))
+        out = np.zeros(self.dims[dataset])
+        if lib().nuts_model_transform_inits(self.h, dataset, d.h, _dp(out)) != 0:
+            raise SamplerError(last_error())
+        return out
+
     def log_prob_grad(self, q, jacobian=True, dataset=0, precision=0):
         q = _f64(np.atleast_2d(q))
         n, D = q.shape

@@ -180,8 +180,15 @@ typedef struct nuts_output {
   int tuned_blocks_per_cu;
 } nuts_output;
 
+/* model_base::transform_inits (the `init=` file path of command_helper.hpp:144-245):
+ * the named constrained values in `inits` to the unconstrained vector out[D];
+ * parameters absent from `inits` are NaN, which nuts_sample draws uniformly in
+ * (-R, R) like services::util::initialize. */
+int nuts_model_transform_inits(nuts_model* m, int dataset, const nuts_data* inits, double* out);
+
 /* Runs warmup then sampling for every chain of every dataset in one batch.
  * init: unconstrained inits [chains][D] or NULL (uniform(-R, R), R = init_radius);
+ *       NaN entries are drawn the same way;
  * inv_metric: per-chain inverse metric [chains][D or D*D] or NULL (identity),
  * the `metric_file=` input. */
 int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const double* inv_metric,

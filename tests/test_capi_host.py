@@ -103,3 +103,19 @@ def test_summary_matches_oracle_multichain():
         o = E.summary(x[p].T)
         ref = [o["mean"], o["mcse"], o["sd"], o["mad"], *o["quantiles"], o["ess_bulk"], o["ess_tail"], o["rhat"]]
         assert np.allclose(s[p], ref, rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_transform_inits_maps_constrained_values():
+    """transform_inits: logit / log / identity per parameter, NaN for the
+    parameters an init file leaves out, errors for values outside the support."""
+    import math
+    bern = ca.Model("bernoulli", {"N": 3, "y": [0, 1, 0]})
+    assert abs(bern.transform_inits({"theta": 0.25})[0] - math.log(0.25 / 0.75)) < 1e-15
+    es = ca.Model("eight_schools_cp", {"J": 3, "y": [1.0, 2.0, 3.0], "sigma": [1.0, 1.0, 2.0]})
+    u = es.transform_inits('{"tau": 2.5, "theta": [0.1, 0.2, 0.3]}')
+    assert np.isnan(u[0]) and abs(u[1] - math.log(2.5)) < 1e-15 and np.allclose(u[2:], [0.1, 0.2, 0.3])
+    with pytest.raises(ca.SamplerError, match="tau"):
+        es.transform_inits({"tau": -1.0})
+    with pytest.raises(ca.SamplerError, match="theta"):
+        bern.transform_inits({"theta": 1.5})

@@ -1,0 +1,57 @@
+"""The CmdStan-grammar executable end to end on the GPU: per-chain Stan CSV
+files (file.hpp:189-212 naming), the header/adaptation/draws/footer layout,
+init files with the per-chain lookup and base-file fallback
+(command_helper.hpp:144-245), and stansummary over the result."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.conftest import load_stan_csv
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "cmdstan_amd", "bin")
+DATA = os.path.join(ROOT, "tests", "golden", "bernoulli.data.json")
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(args, cwd):
+    r = subprocess.run([os.path.join(BIN, "bernoulli")] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    return r
+
+
+def test_two_chains_with_base_init_file(tmp_path):
+    (tmp_path / "init.json").write_text(json.dumps({"theta": 0.3}))
+    r = _run(["sample", "num_chains=2", "data", f"file={DATA}", "init=init.json", "output", "file=out.csv",
+              "refresh=500", "random", "seed=12"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Iteration: 2000 / 2000 [100%]  (Sampling)" in r.stdout + r.stderr
+    thetas = []
+    for k in (1, 2):
+        path = tmp_path / f"out_{k}.csv"
+        text = path.read_text()
+        assert "# Adaptation terminated" in text and "Elapsed Time" in text
+        cols, draws, _ = load_stan_csv(str(path))
+        assert cols[:8] == ["lp__", "accept_stat__", "stepsize__", "treedepth__", "n_leapfrog__", "divergent__",
+                            "energy__", "theta"]
+        assert draws.shape == (1000, 8)
+        thetas.append(draws[:, 7])
+    assert 0.2 < np.mean(thetas) < 0.32
+    s = subprocess.run([os.path.join(BIN, "stansummary"), str(tmp_path / "out_1.csv"), str(tmp_path / "out_2.csv")],
+                       capture_output=True, text=True, timeout=60)
+    assert s.returncode == 0 and "theta" in s.stdout and "2 chains" in s.stdout
+
+
+def test_per_chain_init_files_and_bad_init(tmp_path):
+    for k, v in ((1, 0.2), (2, 0.4)):
+        (tmp_path / f"init_{k}.json").write_text(json.dumps({"theta": v}))
+    r = _run(["sample", "num_chains=2", "data", f"file={DATA}", "init=init.json", "output", "file=o.csv"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    (tmp_path / "bad.json").write_text(json.dumps({"theta": 1.5}))
+    r = _run(["sample", "data", f"file={DATA}", "init=bad.json", "output", "file=b.csv"], tmp_path)
+    assert r.returncode != 0 and "theta" in r.stdout + r.stderr
+    r = _run(["sample", "num_chains=2", "data", f"file={DATA}", "init=missing.json"], tmp_path)
+    assert r.returncode != 0 and "Can't open" in r.stdout + r.stderr

@@ -441,3 +441,17 @@ def test_glm_large_dense_posterior_matches_fp64():
     for j in (1, 50, 140):
         sa, sb = E.summary(a.param(f"beta.{j}")), E.summary(b.param(f"beta.{j}"))
         assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"]), (j, sa, sb)
+
+
+def test_partial_inits_are_used_and_completed():
+    """An init giving some parameters: those start where given, the others are
+    drawn; the first evaluated state is the given one (replay from it)."""
+    data = dict(J=3, y=[1.0, 2.0, 3.0], sigma=[1.0, 1.0, 2.0])
+    m = ca.Model("eight_schools_cp", data)
+    u = m.transform_inits({"mu": 0.5, "tau": 2.0, "theta": [0.1, 0.2, 0.3]})
+    fit = m.sample(num_chains=4, num_warmup=0, num_samples=1, adapt_engaged=False, stepsize=1e-9, init=u, seed=3)
+    x = np.stack(fit.draws)[:, 0, 7:]
+    assert np.allclose(x[:, 0], 0.5, atol=1e-6) and np.allclose(x[:, 1], 2.0, atol=1e-6)
+    part = m.transform_inits({"mu": 0.5})
+    fit2 = m.sample(num_chains=8, num_warmup=20, num_samples=10, init=part, seed=4)
+    assert np.all(fit2.status == 0)

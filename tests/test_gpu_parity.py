@@ -455,3 +455,25 @@ def test_partial_inits_are_used_and_completed():
     part = m.transform_inits({"mu": 0.5})
     fit2 = m.sample(num_chains=8, num_warmup=20, num_samples=10, init=part, seed=4)
     assert np.all(fit2.status == 0)
+
+
+def test_output_rows_thin_save_warmup_and_zero_samples():
+    """Row counts follow generate_transitions: ceil(warmup/thin) saved warmup
+    rows when save_warmup, ceil(samples/thin) sampling rows; zero draws is a
+    valid request."""
+    m = ca.Model("bernoulli", BERN)
+    fit = m.sample(num_chains=3, num_warmup=100, num_samples=50, thin=3, save_warmup=True, seed=2)
+    assert all(d.shape[0] == 34 + 17 for d in fit.draws)
+    fit0 = m.sample(num_chains=2, num_warmup=50, num_samples=0, seed=2)
+    assert all(d.shape[0] == 0 for d in fit0.draws) and np.all(fit0.status == 0)
+
+
+def test_logistic_without_observations_is_the_prior():
+    """N = 0 rows: the batched GLM path returns the prior, beta ~ N(0, 2.5^2)."""
+    D = 3
+    m = ca.Model("logistic_regression", dict(N=0, D=D, X=np.zeros((0, D)), y=np.zeros(0, dtype=np.int64)))
+    fit = m.sample(num_chains=64, num_warmup=300, num_samples=300, seed=5)
+    assert np.all(fit.status == 0)
+    for j in range(1, D + 1):
+        s = E.summary(fit.param(f"beta.{j}"))
+        assert abs(s["mean"]) < 4 * s["mcse"] and abs(s["sd"] - 2.5) < 0.15

@@ -203,8 +203,12 @@ struct NutsWave {
   }
   // T = 1/2 p·Minv p; for dense ps must hold Minv p
   __device__ double kinetic(const double* p, const double* ps) const {
+    const double* __restrict__ pr = p;
+    const double* __restrict__ sr = DENSE ? ps : (METRIC == METRIC_UNIT ? p : minv);
     double a = 0;
-    for (int i = tid; i < D; i += TPC) a += p[i] * sh(p, ps, i); converge();
+#pragma unroll 4
+    for (int i = tid; i < D; i += TPC) a += pr[i] * (DENSE ? sr[i] : (METRIC == METRIC_UNIT ? pr[i] : sr[i] * pr[i]));
+    converge();
     return 0.5 * gsum(a);
   }
 
@@ -523,15 +527,18 @@ struct NutsWave {
       const double* les = DENSE ? LSend(k) : nullptr;
       const double* sbs = DENSE ? (k == 0 ? ps : LSbeg(k - 1)) : nullptr;
       double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+      double* __restrict__ srr = srho;
+      const double* __restrict__ lrr = lr;
+#pragma unroll 4
       for (int i = tid; i < D; i += TPC) {
-        const double rs = lr[i] + srho[i];
-        const double r2 = lr[i] + sb[i];
-        const double r3 = srho[i] + le[i];
+        const double rs = lrr[i] + srr[i];
+        const double r2 = lrr[i] + sb[i];
+        const double r3 = srr[i] + le[i];
         const double slb = sh(lb, lbs, i), sp = sh(p, ps, i);
         a1 += slb * rs; b1 += sp * rs;
         a2 += slb * r2; b2 += sh(sb, sbs, i) * r2;
         a3 += sh(le, les, i) * r3; b3 += sp * r3;
-        srho[i] = rs;
+        srr[i] = rs;
       } converge();
       a1 = gsum(a1); b1 = gsum(b1); a2 = gsum(a2);
       b2 = gsum(b2); a3 = gsum(a3); b3 = gsum(b3);
@@ -586,8 +593,11 @@ struct NutsWave {
     const double* bbs = fwd ? ops : ps;
     const double* ffs = fwd ? ps : ops;
     double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+    double* __restrict__ rhor = rho;
+    const double* __restrict__ srr = srho;
+#pragma unroll 4
     for (int i = tid; i < D; i += TPC) {
-      const double old = rho[i], nw = srho[i];
+      const double old = rhor[i], nw = srr[i];
       const double rn = fwd ? old + nw : nw + old;  // rho_bck + rho_fwd
       const double rb = fwd ? old : nw, rf = fwd ? nw : old;
       const double r2 = rb + pfb[i];
@@ -596,8 +606,9 @@ struct NutsWave {
       a1 += sbb * rn; b1 += sff * rn;
       a2 += sbb * r2; b2 += sh(pfb, psfb, i) * r2;
       a3 += sh(pbf, psbf, i) * r3; b3 += sff * r3;
-      rho[i] = rn;
-    } converge();
+      rhor[i] = rn;
+    }
+    converge();
     a1 = gsum(a1); b1 = gsum(b1); a2 = gsum(a2);
     b2 = gsum(b2); a3 = gsum(a3); b3 = gsum(b3);
     const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
@@ -819,7 +830,9 @@ __global__ void __launch_bounds__(256) dense_factor_kernel(NutsBuffers B, int* f
 template <int METRIC, class GradFn, int TPC = NUTS_WAVE>
 __device__ __forceinline__ void nuts_advance_body(const NutsParams& P, const NutsBuffers& B, const GradFn& grad) {
   __shared__ ChainState sst[4];
-  const int w = threadIdx.x >> 6;
+  // the wave index is uniform over the wave: keep it (and the chain index and
+  // every pointer derived from it) in scalar registers
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int c = TPC == NUTS_WAVE ? blockIdx.x * 4 + w : blockIdx.x;
   const int tid = TPC == NUTS_WAVE ? lane : (int)threadIdx.x;

@@ -22,7 +22,7 @@ struct ModelData {
 struct BernoulliFn {
   static constexpr bool fused = true;
   ModelData md;
-  __device__ double operator()(int c, const double* q, double* gV, int, int lane) const {
+  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const double N = d[0], s = d[1];
     const double u = q[0];
@@ -42,7 +42,7 @@ struct BernoulliFn {
 struct GaussFn {
   static constexpr bool fused = true;
   ModelData md;
-  __device__ double operator()(int c, const double* q, double* gV, int D, int lane) const {
+  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int D, int lane) const {
     const double* d = md.data + md.off[c];
     const double* mu = d + 1;
     const double* P = d + 1 + D;
@@ -69,7 +69,7 @@ struct GaussFn {
 struct EightSchoolsFn {
   static constexpr bool fused = true;
   ModelData md;
-  __device__ double operator()(int c, const double* q, double* gV, int, int lane) const {
+  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const int J = (int)d[0];
     const double mu_s = d[1], tau_s = d[2];
@@ -109,7 +109,7 @@ struct EightSchoolsFn {
 struct HierNormalNCFn {
   static constexpr bool fused = true;
   ModelData md;
-  __device__ double operator()(int c, const double* q, double* gV, int, int lane) const {
+  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const int J = (int)d[0];
     const double mu_s = d[1], tau_s = d[2];
@@ -117,6 +117,7 @@ struct HierNormalNCFn {
     const double* sg = d + 3 + J;
     const double mu = q[0], v = q[1], tau = exp(v);
     double acc = 0, gmu = 0, gtau = 0;
+#pragma unroll 4
     for (int j = lane; j < J; j += NUTS_WAVE) {
       const double eta = q[2 + j];
       const double th = mu + tau * eta;
@@ -152,7 +153,7 @@ struct HierNormalNCFn {
 struct HierLogisticFn {
   static constexpr bool fused = true;
   ModelData md;
-  __device__ double operator()(int c, const double* q, double* gV, int, int lane) const {
+  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const int N = (int)d[0], K = (int)d[1], G = (int)d[2];
     const double mu_s = d[3];
@@ -221,7 +222,7 @@ struct HierLogisticFn {
 struct LogisticFn {
   static constexpr bool fused = true;
   ModelData md;
-  __device__ double operator()(int c, const double* q, double* gV, int D, int lane) const {
+  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int D, int lane) const {
     const double* d = md.data + md.off[c];
     const int N = (int)d[0];
     const double s2inv = d[2];

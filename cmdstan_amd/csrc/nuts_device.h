@@ -64,6 +64,112 @@ __device__ __forceinline__ void wave_sync() {
 // element is always touched by the same thread, and cross-thread reads of
 // vectors (reductions, matvecs, factorisations) go through gsum/gsync, which
 // every wave reaches at the same point because the control flow is identical.
+// Lane-strided vector passes of a leaf.  The vectors arrive as
+// restrict-qualified arguments (pool vectors, Q and G never overlap; two
+// read-only arguments may name the same vector), so the unrolled loops issue
+// the loads of several elements before their stores instead of waiting one
+// memory round trip per element.  Every element keeps the arithmetic and the
+// per-thread accumulation order of the step-by-step form.
+template <int METRIC>
+__device__ __forceinline__ double sharp(const double* __restrict__ x, const double* __restrict__ xs,
+                                        const double* __restrict__ minv, int i) {
+  if (METRIC == METRIC_DENSE) return xs[i];
+  return METRIC == METRIC_UNIT ? x[i] : minv[i] * x[i];
+}
+
+// unit/diag: end_update_p(eps/2), then the kinetic energy's per-thread partial
+// of p·Minv p, with the new p also copied to the subtree's rho and, at the
+// first leaf of a subtree, to its edge momentum
+template <int TPC, bool UNIT, bool EDGE>
+__device__ __forceinline__ double pass_leaf_head(double* __restrict__ p, const double* __restrict__ g,
+                                                 const double* __restrict__ minv, double* __restrict__ rho,
+                                                 double* __restrict__ edge, int D, int tid, double h) {
+  double a = 0;
+#pragma unroll 4
+  for (int i = tid; i < D; i += TPC) {
+    const double pi = p[i] - h * g[i];
+    p[i] = pi;
+    rho[i] = pi;
+    if (EDGE) edge[i] = pi;
+    a += pi * (UNIT ? pi : minv[i] * pi);
+  }
+  return a;
+}
+
+// unit/diag: the closed subtree's checkpoint (first and last momentum, rho),
+// the leaf's (q, g) proposal slot when it is kept, then begin_update_p(eps/2)
+// and update_q(eps) of the next leaf
+template <int TPC, bool UNIT, bool BEG, bool SLOT>
+__device__ __forceinline__ void pass_leaf_tail(double* __restrict__ p, double* __restrict__ q,
+                                               const double* __restrict__ g, const double* __restrict__ minv,
+                                               const double* __restrict__ rho, const double* __restrict__ beg,
+                                               double* __restrict__ lbeg, double* __restrict__ lend,
+                                               double* __restrict__ lrho, double* __restrict__ sq,
+                                               double* __restrict__ sg, int D, int tid, double h, double e) {
+#pragma unroll 4
+  for (int i = tid; i < D; i += TPC) {
+    const double po = p[i], qo = q[i], gi = g[i];
+    lbeg[i] = BEG ? beg[i] : po;
+    lend[i] = po;
+    lrho[i] = rho[i];
+    if (SLOT) { sq[i] = qo; sg[i] = gi; }
+    const double pn = po - h * gi;
+    p[i] = pn;
+    q[i] = qo + e * ((UNIT ? 1.0 : minv[i]) * pn);
+  }
+}
+
+// the three generalized U-turn checks of a merge inside a subtree (rho of the
+// merged subtree written back into srho): partial dot products per thread
+template <int TPC, int METRIC>
+__device__ __forceinline__ void pass_merge(const double* __restrict__ lb, const double* __restrict__ le,
+                                           const double* __restrict__ lr, const double* __restrict__ sb,
+                                           const double* __restrict__ p, double* __restrict__ srho,
+                                           const double* __restrict__ minv, const double* __restrict__ lbs,
+                                           const double* __restrict__ les, const double* __restrict__ sbs,
+                                           const double* __restrict__ ps, int D, int tid, double* acc) {
+  double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+#pragma unroll 4
+  for (int i = tid; i < D; i += TPC) {
+    const double l = lr[i], r = srho[i];
+    const double rs = l + r;
+    const double r2 = l + sb[i];
+    const double r3 = r + le[i];
+    const double slb = sharp<METRIC>(lb, lbs, minv, i), sp = sharp<METRIC>(p, ps, minv, i);
+    a1 += slb * rs; b1 += sp * rs;
+    a2 += slb * r2; b2 += sharp<METRIC>(sb, sbs, minv, i) * r2;
+    a3 += sharp<METRIC>(le, les, minv, i) * r3; b3 += sp * r3;
+    srho[i] = rs;
+  }
+  acc[0] = a1; acc[1] = b1; acc[2] = a2; acc[3] = b2; acc[4] = a3; acc[5] = b3;
+}
+
+// the same checks for the top-level merge of a doubling (rho of the whole
+// trajectory updated in place)
+template <int TPC, int METRIC>
+__device__ __forceinline__ void pass_doubling(bool fwd, double* __restrict__ rho, const double* __restrict__ srho,
+                                              const double* __restrict__ bb, const double* __restrict__ ff,
+                                              const double* __restrict__ pfb, const double* __restrict__ pbf,
+                                              const double* __restrict__ minv, const double* __restrict__ bbs,
+                                              const double* __restrict__ ffs, const double* __restrict__ psfb,
+                                              const double* __restrict__ psbf, int D, int tid, double* acc) {
+  double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+#pragma unroll 4
+  for (int i = tid; i < D; i += TPC) {
+    const double old = rho[i], nw = srho[i];
+    const double rn = fwd ? old + nw : nw + old;  // rho_bck + rho_fwd
+    const double rb = fwd ? old : nw, rf = fwd ? nw : old;
+    const double r2 = rb + pfb[i];
+    const double r3 = rf + pbf[i];
+    const double sbb = sharp<METRIC>(bb, bbs, minv, i), sff = sharp<METRIC>(ff, ffs, minv, i);
+    a1 += sbb * rn; b1 += sff * rn;
+    a2 += sbb * r2; b2 += sharp<METRIC>(pfb, psfb, minv, i) * r2;
+    a3 += sharp<METRIC>(pbf, psbf, minv, i) * r3; b3 += sff * r3;
+    rho[i] = rn;
+  }
+  acc[0] = a1; acc[1] = b1; acc[2] = a2; acc[3] = b2; acc[4] = a3; acc[5] = b3;
+}
+
 template <int METRIC, class GradFn, int TPC = NUTS_WAVE>
 struct NutsWave {
   static_assert(TPC % NUTS_WAVE == 0 && TPC <= 256, "threads per chain");
@@ -490,25 +596,39 @@ struct NutsWave {
   __device__ bool leaf() {
     const double e = s.dir * s.eps;
     s.Vz = pending_V();
-    leap_end(e);
     double* p = vecp(PV_P);
     double* ps = vecp(PV_PS);
+    double* srho = vecp(PV_SRHO);
     s.n_leapfrog++;
-    double h = kinetic(p, ps) + s.Vz;
+    double h;
+    if (DENSE) {
+      leap_end(e);
+      h = kinetic(p, ps) + s.Vz;
+    } else {
+      // leap_end, kinetic, the edge copy and the srho copy in one pass (srho
+      // is written before the divergence check: a diverging transition ends
+      // here and never reads it)
+      constexpr bool UNIT = METRIC == METRIC_UNIT;
+      double* edge = s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF);
+      const double hh = 0.5 * e;
+      const double a = s.leaf_idx == 0 ? pass_leaf_head<TPC, UNIT, true>(p, g, minv, srho, edge, D, tid, hh)
+                                       : pass_leaf_head<TPC, UNIT, false>(p, g, minv, srho, edge, D, tid, hh);
+      converge();
+      h = 0.5 * gsum(a) + s.Vz;
+    }
     if (isnan(h)) h = __builtin_inf();
     if ((h - s.H0) > P.max_deltaH) s.divergent = 1;
     const double lw = s.H0 - h;
     if (s.H0 - h > 0) s.sum_metro += 1;
     else s.sum_metro += exp(s.H0 - h);
     s.leaf_h = h;
-    if (s.leaf_idx == 0) {
+    if (DENSE && s.leaf_idx == 0) {
       copy(s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF), p);
-      if (DENSE) copy(s.z_is_fwd ? vecp(PV_PSFB) : vecp(PV_PSBF), ps);
+      copy(s.z_is_fwd ? vecp(PV_PSFB) : vecp(PV_PSBF), ps);
     }
     if (s.divergent) { trans_end(); return false; }
 
-    double* srho = vecp(PV_SRHO);
-    copy(srho, p);
+    if (DENSE) copy(srho, p);
     s.S_logw = lw;
     s.s_prop = PROP_LEAF;
     int k = 0;
@@ -526,22 +646,11 @@ struct NutsWave {
       const double* lbs = DENSE ? LSbeg(k) : nullptr;
       const double* les = DENSE ? LSend(k) : nullptr;
       const double* sbs = DENSE ? (k == 0 ? ps : LSbeg(k - 1)) : nullptr;
-      double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
-      double* __restrict__ srr = srho;
-      const double* __restrict__ lrr = lr;
-#pragma unroll 4
-      for (int i = tid; i < D; i += TPC) {
-        const double rs = lrr[i] + srr[i];
-        const double r2 = lrr[i] + sb[i];
-        const double r3 = srr[i] + le[i];
-        const double slb = sh(lb, lbs, i), sp = sh(p, ps, i);
-        a1 += slb * rs; b1 += sp * rs;
-        a2 += slb * r2; b2 += sh(sb, sbs, i) * r2;
-        a3 += sh(le, les, i) * r3; b3 += sp * r3;
-        srr[i] = rs;
-      } converge();
-      a1 = gsum(a1); b1 = gsum(b1); a2 = gsum(a2);
-      b2 = gsum(b2); a3 = gsum(a3); b3 = gsum(b3);
+      double acc[6];
+      pass_merge<TPC, METRIC>(lb, le, lr, sb, p, srho, minv, lbs, les, sbs, DENSE ? ps : nullptr, D, tid, acc);
+      converge();
+      const double a1 = gsum(acc[0]), b1 = gsum(acc[1]), a2 = gsum(acc[2]);
+      const double b2 = gsum(acc[3]), a3 = gsum(acc[4]), b3 = gsum(acc[5]);
       const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
       s.S_logw = lsw_sub;
       if (!ok) { free_slot(s.s_prop); trans_end(); return false; }
@@ -549,17 +658,44 @@ struct NutsWave {
     }
     if (s.leaf_idx == (1 << s.depth) - 1) { doubling_end(); return false; }
     // the closed subtree becomes the open left sibling at level k
-    copy(Lbeg(k), k == 0 ? p : Lbeg(k - 1));
-    copy(Lend(k), p);
-    copy(Lrho(k), srho);
     if (DENSE) {
+      copy(Lbeg(k), k == 0 ? p : Lbeg(k - 1));
+      copy(Lend(k), p);
+      copy(Lrho(k), srho);
       copy(LSbeg(k), k == 0 ? ps : LSbeg(k - 1));
       copy(LSend(k), ps);
+      s.L_logw[k] = s.S_logw;
+      s.L_prop[k] = s.s_prop == PROP_LEAF ? store_leaf_slot() : s.s_prop;
+      s.leaf_idx++;
+      leap_begin(e);
+      return request(PH_LEAF);
+    }
+    // unit/diag: the checkpoint copies, the proposal slot and the next leaf's
+    // leap_begin in one pass
+    constexpr bool UNIT = METRIC == METRIC_UNIT;
+    const int slot = s.s_prop == PROP_LEAF ? alloc_slot() : -1;
+    double* lb = Lbeg(k);
+    double* le = Lend(k);
+    double* lr = Lrho(k);
+    const double* beg = k == 0 ? nullptr : Lbeg(k - 1);
+    double* sq = slot >= 0 ? Sq(slot) : nullptr;
+    double* sg = slot >= 0 ? Sg(slot) : nullptr;
+    const double hh = 0.5 * e;
+    if (beg) {
+      if (sq) pass_leaf_tail<TPC, UNIT, true, true>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
+      else pass_leaf_tail<TPC, UNIT, true, false>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
+    } else {
+      if (sq) pass_leaf_tail<TPC, UNIT, false, true>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
+      else pass_leaf_tail<TPC, UNIT, false, false>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
+    }
+    converge();
+    if (slot >= 0) {
+      s.slot_V[slot] = s.Vz;
+      s.slot_H[slot] = s.leaf_h;
     }
     s.L_logw[k] = s.S_logw;
-    s.L_prop[k] = s.s_prop == PROP_LEAF ? store_leaf_slot() : s.s_prop;
+    s.L_prop[k] = slot >= 0 ? slot : s.s_prop;
     s.leaf_idx++;
-    leap_begin(e);
     return request(PH_LEAF);
   }
 
@@ -592,25 +728,11 @@ struct NutsWave {
     const double* ff = fwd ? p : op;   // p_fwd_fwd
     const double* bbs = fwd ? ops : ps;
     const double* ffs = fwd ? ps : ops;
-    double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
-    double* __restrict__ rhor = rho;
-    const double* __restrict__ srr = srho;
-#pragma unroll 4
-    for (int i = tid; i < D; i += TPC) {
-      const double old = rhor[i], nw = srr[i];
-      const double rn = fwd ? old + nw : nw + old;  // rho_bck + rho_fwd
-      const double rb = fwd ? old : nw, rf = fwd ? nw : old;
-      const double r2 = rb + pfb[i];
-      const double r3 = rf + pbf[i];
-      const double sbb = sh(bb, bbs, i), sff = sh(ff, ffs, i);
-      a1 += sbb * rn; b1 += sff * rn;
-      a2 += sbb * r2; b2 += sh(pfb, psfb, i) * r2;
-      a3 += sh(pbf, psbf, i) * r3; b3 += sff * r3;
-      rhor[i] = rn;
-    }
+    double acc[6];
+    pass_doubling<TPC, METRIC>(fwd, rho, srho, bb, ff, pfb, pbf, minv, bbs, ffs, psfb, psbf, D, tid, acc);
     converge();
-    a1 = gsum(a1); b1 = gsum(b1); a2 = gsum(a2);
-    b2 = gsum(b2); a3 = gsum(a3); b3 = gsum(b3);
+    const double a1 = gsum(acc[0]), b1 = gsum(acc[1]), a2 = gsum(acc[2]);
+    const double b2 = gsum(acc[3]), a3 = gsum(acc[4]), b3 = gsum(acc[5]);
     const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
     if (!ok || s.depth >= P.max_depth) trans_end();
     else s.phase = PH_DOUBLING_BEGIN;

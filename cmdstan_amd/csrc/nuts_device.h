@@ -43,9 +43,9 @@ struct ExternalGrad {
 // Makes this wave's earlier global writes visible to later reads by other
 // lanes of the same wave (vectors are written lane-strided and then read whole
 // by every lane in matvecs, the Welford covariance update, the Cholesky
-// columns and the model gradients).  Agent scope: the release waits for the
-// stores, the acquire invalidates the CU's vector L1 so a line cached before
-// the stores is not read back stale.
+// columns and the model gradients).  Work-group scope is enough: writer and
+// reader run on one CU and share its vector L1, so no L2 write-back or L1
+// invalidation is needed (kernels are not built in thread-group split mode).
 // Orders LDS writes of this wave before later LDS reads by its other lanes.
 __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -53,9 +53,9 @@ __device__ __forceinline__ void lds_sync() {
 }
 
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // TPC threads per chain: 64 (one wave) or a multiple of 64 (one block per
@@ -233,9 +233,9 @@ struct NutsWave {
     if constexpr (TPC == NUTS_WAVE) {
       wave_sync();
     } else {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
   }
   static __device__ int gand(int v) {

@@ -112,7 +112,7 @@ def bench_c1(args):
     cpu_ess = E.summary(r["draws"][:, :, 7].T)["ess_bulk"]
     return {"metric": "min bulk-ESS/sec (bernoulli, 4 chains)", "value": ess / fit.sampling_seconds, "unit": "ESS/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True, "dtype": "f64",
+            "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True, "dtype": dtype,
             "data": "examples/bernoulli/bernoulli.data.json",
             "config": {"workload": "bernoulli N=10, 4 chains, diag_e, fused per-chain kernel"},
             "grad_evals_per_sec": float(arr[:, :, 4].sum()) / fit.sampling_seconds,
@@ -146,8 +146,9 @@ def bench_c3(args):
     keep = 10  # mu, log tau, eta[1..8] per chain
     m = ca.Model("hier_normal_nc", datasets)
     print(f"c3: {nm} models, batched", file=sys.stderr, flush=True)
+    # batched: FP64 or the FP32 leapfrog (--precision fp32); model at a time: plain FP64
     fit = m.sample(num_chains=64, num_warmup=args.warmup, num_samples=args.steps, seed=args.seed,
-                   max_saved_params=keep, refresh=50)
+                   max_saved_params=keep, refresh=50, precision=args.precision)
     arr = np.stack(fit.draws)  # chains x draws x (7 + keep)
     C = arr.shape[0]
     ess = []
@@ -157,8 +158,16 @@ def bench_c3(args):
         ess.append(np.nanmin(st[:, 7]))
     min_ess = float(np.min(ess))
     grads = float(arr[:, :, 4].sum())
-    # model at a time on a spread of sizes, extrapolated to all models
     Js = np.array([d["J"] for d in datasets])
+    dtype = "f64" if args.precision == "fp64" else "f32 vectors, f64 sums (FP32 leapfrog)"
+    rhat = max(float(np.nanmax(summarize(arr[64 * k:64 * (k + 1), :, 7:7 + keep].transpose(2, 0, 1))[:, 9]))
+               for k in range(nm))
+    if args.c3_batched_only:
+        return {"metric": "min bulk-ESS/sec, 512 uneven models batched in one launch",
+                "value": min_ess / fit.sampling_seconds, "unit": "ESS/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "dtype": dtype, "grad_evals_per_sec": grads / fit.sampling_seconds,
+                "rhat_max": rhat, "sampling_seconds": fit.sampling_seconds, "warmup_seconds": fit.warmup_seconds}
+    # model at a time on a spread of sizes, extrapolated to all models
     picks = np.argsort(Js)[np.linspace(0, nm - 1, min(8, nm)).astype(int)]
     t_plain = []
     for k in picks:
@@ -173,12 +182,12 @@ def bench_c3(args):
     plain_total = float(np.sum(coef[0] + coef[1] * Js))
     return {"metric": "min bulk-ESS/sec, 512 uneven models batched in one launch", "value": min_ess / fit.sampling_seconds,
             "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True, "dtype": "f64",
+            "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True, "dtype": dtype,
             "data": "synthetic",
             "config": {"workload": f"{nm} non-centered hierarchical normal models, J~U{{8..2000}}, 64 chains each "
                                    f"({C} chains), diag_e, fused per-chain kernel; ESS over mu, log tau, eta[1..8]"},
-            "grad_evals_per_sec": grads / fit.sampling_seconds, "sampling_seconds": fit.sampling_seconds,
-            "warmup_seconds": fit.warmup_seconds, "sum_J": int(Js.sum()),
+            "grad_evals_per_sec": grads / fit.sampling_seconds, "rhat_max": rhat,
+            "sampling_seconds": fit.sampling_seconds, "warmup_seconds": fit.warmup_seconds, "sum_J": int(Js.sum()),
             "plain_model_at_a_time": {"sampling_seconds_extrapolated": plain_total,
                                       "min_bulk_ess_per_sec": min_ess / plain_total,
                                       "measured_models": [int(Js[k]) for k in picks],
@@ -307,6 +316,7 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default): the headline workload; c1, c3: secondary measurements")
     ap.add_argument("--models", type=int, default=512, help="c3: number of models")
+    ap.add_argument("--c3-batched-only", action="store_true", help="c3: skip the model-at-a-time comparison")
     ap.add_argument("--flow-steps", type=int, default=1000, help="c4: Adam steps")
     ap.add_argument("--N5", type=int, default=1000000, help="c5: observations")
     ap.add_argument("--D5", type=int, default=1000, help="c5: coefficients")

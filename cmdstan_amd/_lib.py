@@ -63,6 +63,8 @@ SIGNATURES = {
     "nuts_sample": (C.c_int, [C.c_void_p, C.POINTER(nuts_config), DP, DP, C.POINTER(nuts_output)]),
     "nuts_transition_replay": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, DP,
                                          DP, DP, C.c_int64, DP, C.c_int64, DP, DP]),
+    "nuts_transition_replay_precision": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                   C.c_int, DP, DP, DP, C.c_int64, DP, C.c_int64, C.c_int, DP, DP]),
     "nuts_summary": (C.c_int, [DP, C.c_int, C.c_int, C.c_int, C.c_int, DP]),
     "nuts_flow_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
     "nuts_flow_destroy": (None, [C.c_void_p]),

@@ -510,8 +510,8 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   HIPC(hipMemsetAsync(pool.p, 0, (size_t)tot * NV * sizeof(double), st));
   HIPC(hipMemsetAsync(dneed.p, 0, C * sizeof(int), st));
   HIPC(hipMemsetAsync(dstop.p, 0, sizeof(int), st));
+  std::vector<double> hm(tot, 1.0);  // diagonal inverse metric as uploaded
   {
-    std::vector<double> hm(tot, 1.0);
     if (R.inv_metric && cfg.metric == NUTS_METRIC_DIAG_E) {
       for (int c = 0; c < C; ++c)
         for (int i = 0; i < D[c]; ++i) {
@@ -521,7 +521,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         }
     }
     minv.upload(hm, st);
-    HIPC(hipStreamSynchronize(st));  // hm is released at the end of this scope
+    HIPC(hipStreamSynchronize(st));
   }
   DevBuf<double> dMb, dLb, dWb;
   if (dense) {
@@ -635,6 +635,19 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     fused = false;
   }
   if (!fused && M.kind == MK_LOGISTIC && !glm_ok) fused = false;  // per-chain gradient kernel in lockstep
+  // FP32 leapfrog: the fused per-chain kernel keeps q, p, g, the checkpoints and
+  // the metric in FP32 (arithmetic, H, tree weights and U-turn sums in FP64)
+  const bool vec32 = fused && cfg.precision == 1 && !M.glm && !M.glm_large;
+  if (vec32 && (dense || !model_supports_vec32(M.kind)))
+    throw std::invalid_argument("precision=fp32 on the per-chain kernel needs unit_e or diag_e and a model with FP32 vectors");
+  if (fused && cfg.precision == 2 && !M.glm && !M.glm_large)
+    throw std::invalid_argument("precision=bf16 applies to the logistic regression design matrix only");
+  if (vec32) {
+    // same element offsets, FP32 values in the first half of each buffer
+    std::vector<float> hf(hm.begin(), hm.end());
+    HIPC(hipMemcpyAsync(minv.p, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+  }
 
   DevBuf<double> work;
   DevBuf<int> act_idx, n_act;
@@ -792,7 +805,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
       if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() > max_seconds)
         throw std::runtime_error("sampler did not finish within CMDSTAN_AMD_MAX_SECONDS");
       if (fused) {
-        launch_advance_fused(M.kind, cfg.metric, P, B, md, st);
+        launch_advance_fused(M.kind, cfg.metric, vec32, P, B, md, st);
       } else if (use_graph) {
         if (!graph.x) {
           HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -871,11 +884,16 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     for (int c = 0; c < C; ++c)
       for (int i = 0; i < D[c]; ++i)
         for (int j = 0; j < D[c]; ++j) res.minv.push_back(hm[moff[c] + (int64_t)j * D[c] + i]);
-  } else {
-    std::vector<double> hm(tot);
-    HIPC(hipMemcpyAsync(hm.data(), minv.p, tot * sizeof(double), hipMemcpyDeviceToHost, st));
+  } else if (vec32) {
+    std::vector<float> hf(tot);
+    HIPC(hipMemcpyAsync(hf.data(), minv.p, tot * sizeof(float), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
-    for (int c = 0; c < C; ++c) for (int i = 0; i < D[c]; ++i) res.minv.push_back(hm[off[c] + i]);
+    for (int c = 0; c < C; ++c) for (int i = 0; i < D[c]; ++i) res.minv.push_back(hf[off[c] + i]);
+  } else {
+    std::vector<double> hmo(tot);
+    HIPC(hipMemcpyAsync(hmo.data(), minv.p, tot * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    for (int c = 0; c < C; ++c) for (int i = 0; i < D[c]; ++i) res.minv.push_back(hmo[off[c] + i]);
   }
   if (rows > 0 && (msp < 0 || msp >= D[0])) {
     bool uni = true;
@@ -1408,9 +1426,19 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
 int nuts_transition_replay(nuts_model* m, int dataset, int num_chains, int metric, double stepsize, double jitter,
                            int max_depth, const double* inv_metric, const double* q0, const double* u, int64_t n_u,
                            const double* z, int64_t n_z, double* out_q, double* info) {
+  return nuts_transition_replay_precision(m, dataset, num_chains, metric, stepsize, jitter, max_depth, inv_metric, q0,
+                                          u, n_u, z, n_z, 0, out_q, info);
+}
+
+int nuts_transition_replay_precision(nuts_model* m, int dataset, int num_chains, int metric, double stepsize,
+                                     double jitter, int max_depth, const double* inv_metric, const double* q0,
+                                     const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,
+                                     double* out_q, double* info) {
   return guarded([&]() {
     RunSpec R;
     nuts_config_default(&R.cfg);
+    R.cfg.precision = precision;
+    validate(R.cfg);
     R.cfg.num_warmup = 0; R.cfg.num_samples = 1; R.cfg.adapt_engaged = 0;
     R.cfg.metric = metric; R.cfg.stepsize = stepsize; R.cfg.stepsize_jitter = jitter;
     R.cfg.max_depth = max_depth; R.cfg.num_chains = num_chains; R.cfg.fused = 1;

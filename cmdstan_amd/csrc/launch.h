@@ -31,8 +31,11 @@ void launch_advance_external(int metric, const NutsParams& P, const NutsBuffers&
 void launch_advance_external_block(int metric, const NutsParams& P, const NutsBuffers& B, hipStream_t st);
 // Cholesky factor of every chain's dense inverse metric (B.dM -> B.dL); counts failures.
 void launch_dense_factor(const NutsBuffers& B, int* failed, hipStream_t st);
-void launch_advance_fused(int kind, int metric, const NutsParams& P, const NutsBuffers& B, const ModelDataDev& md,
-                          hipStream_t st);
+// vec32: FP32 vector storage (unit/diag, models that support it); sums, H, U-turn
+// dot products and tree weights stay FP64
+void launch_advance_fused(int kind, int metric, bool vec32, const NutsParams& P, const NutsBuffers& B,
+                          const ModelDataDev& md, hipStream_t st);
+bool model_supports_vec32(int kind);
 
 // Per-chain model gradient: V[c], G row c at Q row c for chains with need[c]
 // (need == nullptr: all chains).  Clears need[c].

@@ -21,8 +21,10 @@ struct ModelData {
 // block = [N, sum(y)], q = logit(theta)
 struct BernoulliFn {
   static constexpr bool fused = true;
+  static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
-  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
+  template <class T>
+  __device__ double operator()(int c, const T* __restrict__ q, T* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const double N = d[0], s = d[1];
     const double u = q[0];
@@ -41,8 +43,10 @@ struct BernoulliFn {
 // q ~ multi_normal_prec(mu, P); block = [D, mu[D], P[D*D] row-major]
 struct GaussFn {
   static constexpr bool fused = true;
+  static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
-  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int D, int lane) const {
+  template <class T>
+  __device__ double operator()(int c, const T* __restrict__ q, T* __restrict__ gV, int D, int lane) const {
     const double* d = md.data + md.off[c];
     const double* mu = d + 1;
     const double* P = d + 1 + D;
@@ -68,8 +72,10 @@ struct GaussFn {
 // block = [J, mu_s, tau_s, y[J], sigma[J]]
 struct EightSchoolsFn {
   static constexpr bool fused = true;
+  static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
-  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
+  template <class T>
+  __device__ double operator()(int c, const T* __restrict__ q, T* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const int J = (int)d[0];
     const double mu_s = d[1], tau_s = d[2];
@@ -108,8 +114,10 @@ struct EightSchoolsFn {
 // block = [J, mu_s, tau_s, y[J], sigma[J]]; q = (mu, log tau, eta); outputs mu, tau, eta, theta
 struct HierNormalNCFn {
   static constexpr bool fused = true;
+  static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
-  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
+  template <class T>
+  __device__ double operator()(int c, const T* __restrict__ q, T* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const int J = (int)d[0];
     const double mu_s = d[1], tau_s = d[2];
@@ -152,29 +160,31 @@ struct HierNormalNCFn {
 // block = [N, K, G, mu_s, gstart[G+1], X[N*K], y[N]]; q = (mu[K], log tau[K], beta[G][K])
 struct HierLogisticFn {
   static constexpr bool fused = true;
+  static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
-  __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int, int lane) const {
+  template <class T>
+  __device__ double operator()(int c, const T* __restrict__ q, T* __restrict__ gV, int, int lane) const {
     const double* d = md.data + md.off[c];
     const int N = (int)d[0], K = (int)d[1], G = (int)d[2];
     const double mu_s = d[3];
     const double* gstart = d + 4;
     const double* X = d + 5 + G;
     const double* y = X + (int64_t)N * K;
-    const double* mu = q;
-    const double* vv = q + K;
-    const double* beta = q + 2 * K;
-    double* gb = gV + 2 * K;
+    const T* mu = q;
+    const T* vv = q + K;
+    const T* beta = q + 2 * K;
+    T* gb = gV + 2 * K;
     double acc = 0;
     // groups: one lane owns whole groups, so the beta gradient needs no atomics
     for (int gi = lane; gi < G; gi += NUTS_WAVE) {
-      const double* b = beta + (int64_t)gi * K;
+      const T* b = beta + (int64_t)gi * K;
       double gacc[16];
       for (int k = 0; k < K && k < 16; ++k) {
-        const double tk = exp(vv[k]);
-        const double dd = b[k] - mu[k];
+        const double tk = exp((double)vv[k]);
+        const double dd = (double)b[k] - (double)mu[k];
         gacc[k] = -dd / (tk * tk);
         const double z = dd / tk;
-        acc += -vv[k] - 0.5 * z * z;
+        acc += -(double)vv[k] - 0.5 * z * z;
       }
       const int n0 = (int)gstart[gi], n1 = (int)gstart[gi + 1];
       for (int n = n0; n < n1; ++n) {
@@ -189,15 +199,15 @@ struct HierLogisticFn {
     }
     // hyperparameters: lane k owns mu[k], tau[k]
     for (int k = lane; k < K; k += NUTS_WAVE) {
-      const double tk = exp(vv[k]);
-      double gm = -mu[k] / (mu_s * mu_s);
+      const double tk = exp((double)vv[k]);
+      double gm = -(double)mu[k] / (mu_s * mu_s);
       double gt = -tk;
       for (int gi = 0; gi < G; ++gi) {
-        const double dd = beta[(int64_t)gi * K + k] - mu[k];
+        const double dd = (double)beta[(int64_t)gi * K + k] - (double)mu[k];
         gm += dd / (tk * tk);
         gt += -1.0 / tk + dd * dd / (tk * tk * tk);
       }
-      acc += -0.5 * mu[k] * mu[k] / (mu_s * mu_s) - 0.5 * tk * tk + md.jac * vv[k];
+      acc += -0.5 * (double)mu[k] * (double)mu[k] / (mu_s * mu_s) - 0.5 * tk * tk + md.jac * (double)vv[k];
       gV[k] = -gm;
       gV[K + k] = -(md.jac + tk * gt);
     }
@@ -221,6 +231,7 @@ struct HierLogisticFn {
 // block = [N, D, 1/s^2, X[N*D], y[N]]
 struct LogisticFn {
   static constexpr bool fused = true;
+  static constexpr bool vec32 = false;
   ModelData md;
   __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int D, int lane) const {
     const double* d = md.data + md.off[c];

@@ -33,7 +33,8 @@ namespace nutsgpu {
 
 struct ExternalGrad {
   static constexpr bool fused = false;
-  __device__ double operator()(int, const double*, double*, int, int) const { return 0.0; }
+  template <class T>
+  __device__ double operator()(int, const T*, T*, int, int) const { return 0.0; }
 };
 
 // dense_e keeps one momentum in an LDS strip of the wave during the
@@ -70,28 +71,29 @@ __device__ __forceinline__ void wave_sync() {
 // the loads of several elements before their stores instead of waiting one
 // memory round trip per element.  Every element keeps the arithmetic and the
 // per-thread accumulation order of the step-by-step form.
-template <int METRIC>
-__device__ __forceinline__ double sharp(const double* __restrict__ x, const double* __restrict__ xs,
-                                        const double* __restrict__ minv, int i) {
+template <int METRIC, class VT>
+__device__ __forceinline__ double sharp(const VT* __restrict__ x, const VT* __restrict__ xs,
+                                        const VT* __restrict__ minv, int i) {
   if (METRIC == METRIC_DENSE) return xs[i];
-  return METRIC == METRIC_UNIT ? x[i] : minv[i] * x[i];
+  return METRIC == METRIC_UNIT ? (double)x[i] : (double)minv[i] * (double)x[i];
 }
 
 // unit/diag: end_update_p(eps/2), then the kinetic energy's per-thread partial
 // of p·Minv p, with the new p also copied to the subtree's rho and, at the
 // first leaf of a subtree, to its edge momentum
-template <int TPC, bool UNIT, bool EDGE>
-__device__ __forceinline__ double pass_leaf_head(double* __restrict__ p, const double* __restrict__ g,
-                                                 const double* __restrict__ minv, double* __restrict__ rho,
-                                                 double* __restrict__ edge, int D, int tid, double h) {
+template <int TPC, bool UNIT, bool EDGE, class VT>
+__device__ __forceinline__ double pass_leaf_head(VT* __restrict__ p, const VT* __restrict__ g,
+                                                 const VT* __restrict__ minv, VT* __restrict__ rho,
+                                                 VT* __restrict__ edge, int D, int tid, double h) {
   double a = 0;
 #pragma unroll 4
   for (int i = tid; i < D; i += TPC) {
-    const double pi = p[i] - h * g[i];
-    p[i] = pi;
-    rho[i] = pi;
-    if (EDGE) edge[i] = pi;
-    a += pi * (UNIT ? pi : minv[i] * pi);
+    const VT pv = (double)p[i] - h * (double)g[i];
+    p[i] = pv;
+    rho[i] = pv;
+    if (EDGE) edge[i] = pv;
+    const double pi = pv;
+    a += pi * (UNIT ? pi : (double)minv[i] * pi);
   }
   return a;
 }
@@ -99,91 +101,95 @@ __device__ __forceinline__ double pass_leaf_head(double* __restrict__ p, const d
 // unit/diag: the closed subtree's checkpoint (first and last momentum, rho),
 // the leaf's (q, g) proposal slot when it is kept, then begin_update_p(eps/2)
 // and update_q(eps) of the next leaf
-template <int TPC, bool UNIT, bool BEG, bool SLOT>
-__device__ __forceinline__ void pass_leaf_tail(double* __restrict__ p, double* __restrict__ q,
-                                               const double* __restrict__ g, const double* __restrict__ minv,
-                                               const double* __restrict__ rho, const double* __restrict__ beg,
-                                               double* __restrict__ lbeg, double* __restrict__ lend,
-                                               double* __restrict__ lrho, double* __restrict__ sq,
-                                               double* __restrict__ sg, int D, int tid, double h, double e) {
+template <int TPC, bool UNIT, bool BEG, bool SLOT, class VT>
+__device__ __forceinline__ void pass_leaf_tail(VT* __restrict__ p, VT* __restrict__ q, const VT* __restrict__ g,
+                                               const VT* __restrict__ minv, const VT* __restrict__ rho,
+                                               const VT* __restrict__ beg, VT* __restrict__ lbeg,
+                                               VT* __restrict__ lend, VT* __restrict__ lrho, VT* __restrict__ sq,
+                                               VT* __restrict__ sg, int D, int tid, double h, double e) {
 #pragma unroll 4
   for (int i = tid; i < D; i += TPC) {
-    const double po = p[i], qo = q[i], gi = g[i];
+    const VT po = p[i], qo = q[i], gi = g[i];
     lbeg[i] = BEG ? beg[i] : po;
     lend[i] = po;
     lrho[i] = rho[i];
     if (SLOT) { sq[i] = qo; sg[i] = gi; }
-    const double pn = po - h * gi;
+    const VT pn = (double)po - h * (double)gi;
     p[i] = pn;
-    q[i] = qo + e * ((UNIT ? 1.0 : minv[i]) * pn);
+    q[i] = (double)qo + e * ((UNIT ? 1.0 : (double)minv[i]) * (double)pn);
   }
 }
 
 // the three generalized U-turn checks of a merge inside a subtree (rho of the
 // merged subtree written back into srho): partial dot products per thread
-template <int TPC, int METRIC>
-__device__ __forceinline__ void pass_merge(const double* __restrict__ lb, const double* __restrict__ le,
-                                           const double* __restrict__ lr, const double* __restrict__ sb,
-                                           const double* __restrict__ p, double* __restrict__ srho,
-                                           const double* __restrict__ minv, const double* __restrict__ lbs,
-                                           const double* __restrict__ les, const double* __restrict__ sbs,
-                                           const double* __restrict__ ps, int D, int tid, double* acc) {
+template <int TPC, int METRIC, class VT>
+__device__ __forceinline__ void pass_merge(const VT* __restrict__ lb, const VT* __restrict__ le,
+                                           const VT* __restrict__ lr, const VT* __restrict__ sb,
+                                           const VT* __restrict__ p, VT* __restrict__ srho,
+                                           const VT* __restrict__ minv, const VT* __restrict__ lbs,
+                                           const VT* __restrict__ les, const VT* __restrict__ sbs,
+                                           const VT* __restrict__ ps, int D, int tid, double* acc) {
   double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
 #pragma unroll 4
   for (int i = tid; i < D; i += TPC) {
     const double l = lr[i], r = srho[i];
-    const double rs = l + r;
-    const double r2 = l + sb[i];
-    const double r3 = r + le[i];
+    const VT rsv = l + r;
+    const double rs = rsv;
+    const double r2 = l + (double)sb[i];
+    const double r3 = r + (double)le[i];
     const double slb = sharp<METRIC>(lb, lbs, minv, i), sp = sharp<METRIC>(p, ps, minv, i);
     a1 += slb * rs; b1 += sp * rs;
     a2 += slb * r2; b2 += sharp<METRIC>(sb, sbs, minv, i) * r2;
     a3 += sharp<METRIC>(le, les, minv, i) * r3; b3 += sp * r3;
-    srho[i] = rs;
+    srho[i] = rsv;
   }
   acc[0] = a1; acc[1] = b1; acc[2] = a2; acc[3] = b2; acc[4] = a3; acc[5] = b3;
 }
 
 // the same checks for the top-level merge of a doubling (rho of the whole
 // trajectory updated in place)
-template <int TPC, int METRIC>
-__device__ __forceinline__ void pass_doubling(bool fwd, double* __restrict__ rho, const double* __restrict__ srho,
-                                              const double* __restrict__ bb, const double* __restrict__ ff,
-                                              const double* __restrict__ pfb, const double* __restrict__ pbf,
-                                              const double* __restrict__ minv, const double* __restrict__ bbs,
-                                              const double* __restrict__ ffs, const double* __restrict__ psfb,
-                                              const double* __restrict__ psbf, int D, int tid, double* acc) {
+template <int TPC, int METRIC, class VT>
+__device__ __forceinline__ void pass_doubling(bool fwd, VT* __restrict__ rho, const VT* __restrict__ srho,
+                                              const VT* __restrict__ bb, const VT* __restrict__ ff,
+                                              const VT* __restrict__ pfb, const VT* __restrict__ pbf,
+                                              const VT* __restrict__ minv, const VT* __restrict__ bbs,
+                                              const VT* __restrict__ ffs, const VT* __restrict__ psfb,
+                                              const VT* __restrict__ psbf, int D, int tid, double* acc) {
   double a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
 #pragma unroll 4
   for (int i = tid; i < D; i += TPC) {
     const double old = rho[i], nw = srho[i];
-    const double rn = fwd ? old + nw : nw + old;  // rho_bck + rho_fwd
+    const VT rnv = fwd ? old + nw : nw + old;  // rho_bck + rho_fwd
+    const double rn = rnv;
     const double rb = fwd ? old : nw, rf = fwd ? nw : old;
-    const double r2 = rb + pfb[i];
-    const double r3 = rf + pbf[i];
+    const double r2 = rb + (double)pfb[i];
+    const double r3 = rf + (double)pbf[i];
     const double sbb = sharp<METRIC>(bb, bbs, minv, i), sff = sharp<METRIC>(ff, ffs, minv, i);
     a1 += sbb * rn; b1 += sff * rn;
     a2 += sbb * r2; b2 += sharp<METRIC>(pfb, psfb, minv, i) * r2;
     a3 += sharp<METRIC>(pbf, psbf, minv, i) * r3; b3 += sff * r3;
-    rho[i] = rn;
+    rho[i] = rnv;
   }
   acc[0] = a1; acc[1] = b1; acc[2] = a2; acc[3] = b2; acc[4] = a3; acc[5] = b3;
 }
 
-template <int METRIC, class GradFn, int TPC = NUTS_WAVE>
+template <int METRIC, class GradFn, int TPC = NUTS_WAVE, class VT = double>
 struct NutsWave {
   static_assert(TPC % NUTS_WAVE == 0 && TPC <= 256, "threads per chain");
   static constexpr bool DENSE = METRIC == METRIC_DENSE;
+  static_assert(!DENSE || sizeof(VT) == sizeof(double), "dense_e keeps FP64 vectors");
   const NutsParams& P;
   const NutsBuffers& B;
   ChainState& s;
   const GradFn& grad;
   int c, lane, tid, D, Dp;  // lane in the wave, tid in the chain's thread group
   uint32_t uid;
-  double* q;     // Z end position (dense Q row)
-  double* g;     // Z end potential gradient (dense G row)
-  double* blk;   // pool block of this chain
-  double* minv;  // diagonal inverse metric (unit/diag)
+  // vectors are stored as VT (double, or float for the FP32 leapfrog) and
+  // combined in FP64 arithmetic
+  VT* q;     // Z end position (dense Q row)
+  VT* g;     // Z end potential gradient (dense G row)
+  VT* blk;   // pool block of this chain
+  VT* minv;  // diagonal inverse metric (unit/diag)
   double* dM;    // dense: inverse metric, column-major D×D
   double* dL;    // dense: lower Cholesky factor, column-major
   double* dW;    // dense: Welford m2, column-major
@@ -199,10 +205,11 @@ struct NutsWave {
     D = B.D[c];
     Dp = B.Dp[c];
     uid = B.uid[c];
-    q = B.Q + B.off[c];
-    g = B.G + B.off[c];
-    blk = B.pool + B.off[c] * (int64_t)P.NV;
-    minv = B.minv + B.off[c];
+    // FP32 vectors use the first half of each buffer at the same element offsets
+    q = reinterpret_cast<VT*>(B.Q) + B.off[c];
+    g = reinterpret_cast<VT*>(B.G) + B.off[c];
+    blk = reinterpret_cast<VT*>(B.pool) + B.off[c] * (int64_t)P.NV;
+    minv = reinterpret_cast<VT*>(B.minv) + B.off[c];
     if (DENSE) {
       dM = B.dM + B.moff[c];
       dL = B.dL + B.moff[c];
@@ -263,31 +270,31 @@ struct NutsWave {
     else __syncthreads();
   }
 
-  __device__ double* vecp(int id) const { return blk + (int64_t)id * Dp; }
-  __device__ double* Lbeg(int k) const { return vecp(PV_FIXED + k); }
-  __device__ double* Lend(int k) const { return vecp(PV_FIXED + P.maxd + k); }
-  __device__ double* Lrho(int k) const { return vecp(PV_FIXED + 2 * P.maxd + k); }
-  __device__ double* Sq(int k) const { return vecp(PV_FIXED + 3 * P.maxd + k); }
-  __device__ double* Sg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + P.nslots + k); }
-  __device__ double* LSbeg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + 2 * P.nslots + k); }
-  __device__ double* LSend(int k) const { return vecp(PV_FIXED + 4 * P.maxd + 2 * P.nslots + k); }
+  __device__ VT* vecp(int id) const { return blk + (int64_t)id * Dp; }
+  __device__ VT* Lbeg(int k) const { return vecp(PV_FIXED + k); }
+  __device__ VT* Lend(int k) const { return vecp(PV_FIXED + P.maxd + k); }
+  __device__ VT* Lrho(int k) const { return vecp(PV_FIXED + 2 * P.maxd + k); }
+  __device__ VT* Sq(int k) const { return vecp(PV_FIXED + 3 * P.maxd + k); }
+  __device__ VT* Sg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + P.nslots + k); }
+  __device__ VT* LSbeg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + 2 * P.nslots + k); }
+  __device__ VT* LSend(int k) const { return vecp(PV_FIXED + 4 * P.maxd + 2 * P.nslots + k); }
 
   // ------------------------------------------------------------- metric
-  __device__ double mi(int i) const { return METRIC == METRIC_UNIT ? 1.0 : minv[i]; }
+  __device__ double mi(int i) const { return METRIC == METRIC_UNIT ? 1.0 : (double)minv[i]; }
   // sharp momentum element: x = momentum, xs = its stored companion (dense)
-  __device__ double sh(const double* x, const double* xs, int i) const {
+  __device__ double sh(const VT* x, const VT* xs, int i) const {
     if (DENSE) return xs[i];
-    return METRIC == METRIC_UNIT ? x[i] : minv[i] * x[i];
+    return METRIC == METRIC_UNIT ? (double)x[i] : (double)minv[i] * (double)x[i];
   }
   // out = Minv x (dense), summing j = 0..D-1 in order like Eigen's row product
-  __device__ void matvec(const double* x, double* out) const {
+  __device__ void matvec(const VT* x, VT* out) const {
     gsync();
     constexpr int RB = 8;
     for (int i0 = 0; i0 < D; i0 += TPC * RB) {
       double acc[RB];
 #pragma unroll
       for (int r = 0; r < RB; ++r) acc[r] = 0.0;
-      const double* __restrict__ xr = x;
+      const VT* __restrict__ xr = x;
       const double* __restrict__ Mr = dM;
 #pragma unroll 4
       for (int j = 0; j < D; ++j) {
@@ -308,12 +315,15 @@ struct NutsWave {
     gsync();
   }
   // T = 1/2 p·Minv p; for dense ps must hold Minv p
-  __device__ double kinetic(const double* p, const double* ps) const {
-    const double* __restrict__ pr = p;
-    const double* __restrict__ sr = DENSE ? ps : (METRIC == METRIC_UNIT ? p : minv);
+  __device__ double kinetic(const VT* p, const VT* ps) const {
+    const VT* __restrict__ pr = p;
+    const VT* __restrict__ sr = DENSE ? ps : (METRIC == METRIC_UNIT ? p : minv);
     double a = 0;
 #pragma unroll 4
-    for (int i = tid; i < D; i += TPC) a += pr[i] * (DENSE ? sr[i] : (METRIC == METRIC_UNIT ? pr[i] : sr[i] * pr[i]));
+    for (int i = tid; i < D; i += TPC) {
+      const double pv = pr[i];
+      a += pv * (DENSE ? (double)sr[i] : (METRIC == METRIC_UNIT ? pv : (double)sr[i] * pv));
+    }
     converge();
     return 0.5 * gsum(a);
   }
@@ -334,11 +344,11 @@ struct NutsWave {
   __device__ void advance_normals() { s.nctr += B.inj_z ? (uint64_t)D : (uint64_t)((D + 1) / 2); }
 
   // {unit,diag,dense}_e_metric::sample_p; for dense also sets ps = Minv p
-  __device__ void sample_p(double* p, double* ps) {
+  __device__ void sample_p(VT* p, VT* ps) {
     if (!DENSE) {
       for (int i = tid; i < D; i += TPC) {
         const double z = normal_draw(i);
-        p[i] = METRIC == METRIC_UNIT ? z : z / sqrt(minv[i]);
+        p[i] = METRIC == METRIC_UNIT ? z : z / sqrt((double)minv[i]);
       } converge();
       advance_normals();
       return;
@@ -369,43 +379,43 @@ struct NutsWave {
   // ------------------------------------------------------------- vectors
   // the pool vectors never alias each other: restrict lets the unrolled loops
   // issue several independent loads before their stores
-  __device__ void copy(double* __restrict__ dst, const double* __restrict__ src) const {
+  __device__ void copy(VT* __restrict__ dst, const VT* __restrict__ src) const {
 #pragma unroll 4
     for (int i = tid; i < D; i += TPC) dst[i] = src[i];
     converge();
   }
-  __device__ void swap(double* a, double* b) const {
-    for (int i = tid; i < D; i += TPC) { double t = a[i]; a[i] = b[i]; b[i] = t; } converge();
+  __device__ void swap(VT* a, VT* b) const {
+    for (int i = tid; i < D; i += TPC) { const VT t = a[i]; a[i] = b[i]; b[i] = t; } converge();
   }
   // expl_leapfrog: begin_update_p(eps/2), update_q(eps)
   __device__ void leap_begin(double e) const {
-    double* p = vecp(PV_P);
+    VT* p = vecp(PV_P);
     const double h = 0.5 * e;
     if (DENSE) {
-      double* t = vecp(PV_TMP);
+      VT* t = vecp(PV_TMP);
       for (int i = tid; i < D; i += TPC) p[i] = p[i] - h * g[i]; converge();
       matvec(p, t);
       for (int i = tid; i < D; i += TPC) q[i] = q[i] + e * t[i]; converge();
       return;
     }
-    double* __restrict__ pr = p;
-    double* __restrict__ qr = q;
-    const double* __restrict__ gr = g;
+    VT* __restrict__ pr = p;
+    VT* __restrict__ qr = q;
+    const VT* __restrict__ gr = g;
 #pragma unroll 4
     for (int i = tid; i < D; i += TPC) {
-      const double pi = pr[i] - h * gr[i];
+      const VT pi = (double)pr[i] - h * (double)gr[i];
       pr[i] = pi;
-      qr[i] = qr[i] + e * (mi(i) * pi);
+      qr[i] = (double)qr[i] + e * (mi(i) * (double)pi);
     }
     converge();
   }
   // end_update_p(eps/2); dense: also the leaf's sharp momentum
   __device__ void leap_end(double e) const {
-    double* __restrict__ p = vecp(PV_P);
-    const double* __restrict__ gr = g;
+    VT* __restrict__ p = vecp(PV_P);
+    const VT* __restrict__ gr = g;
     const double h = 0.5 * e;
 #pragma unroll 4
-    for (int i = tid; i < D; i += TPC) p[i] = p[i] - h * gr[i];
+    for (int i = tid; i < D; i += TPC) p[i] = (double)p[i] - h * (double)gr[i];
     converge();
     if (DENSE) matvec(p, vecp(PV_PS));
   }
@@ -487,8 +497,8 @@ struct NutsWave {
     copy(q, Sq(k));
     copy(g, Sg(k));
     s.Vz = s.slot_V[k];
-    double* p = vecp(PV_P);
-    double* ps = vecp(PV_PS);
+    VT* p = vecp(PV_P);
+    VT* ps = vecp(PV_PS);
     sample_p(p, ps);
     s.ss_H0 = kinetic(p, ps) + s.Vz;
     leap_begin(s.nom_eps);
@@ -539,22 +549,22 @@ struct NutsWave {
     // (a transition that ended inside a subtree leaves its checkpoints behind)
     const uint32_t all = P.nslots >= 32 ? 0xFFFFFFFFu : ((1u << P.nslots) - 1u);
     s.free_mask = all & ~(1u << k);
-    double* p = vecp(PV_P);
-    double* ps = vecp(PV_PS);
+    VT* p = vecp(PV_P);
+    VT* ps = vecp(PV_PS);
     copy(q, Sq(k));
     copy(g, Sg(k));
     s.Vz = s.slot_V[k];
     sample_p(p, ps);
-    double* oq = vecp(PV_OQ); double* op = vecp(PV_OP); double* og = vecp(PV_OG);
-    double* pfb = vecp(PV_PFB); double* pbf = vecp(PV_PBF); double* rho = vecp(PV_RHO);
+    VT* oq = vecp(PV_OQ); VT* op = vecp(PV_OP); VT* og = vecp(PV_OG);
+    VT* pfb = vecp(PV_PFB); VT* pbf = vecp(PV_PBF); VT* rho = vecp(PV_RHO);
     for (int i = tid; i < D; i += TPC) {
-      const double pi = p[i];
+      const VT pi = p[i];
       oq[i] = q[i]; op[i] = pi; og[i] = g[i];
       pfb[i] = pi; pbf[i] = pi; rho[i] = pi;
     } converge();
     if (DENSE) {
-      double* ops = vecp(PV_OPS); double* psfb = vecp(PV_PSFB); double* psbf = vecp(PV_PSBF);
-      for (int i = tid; i < D; i += TPC) { const double v = ps[i]; ops[i] = v; psfb[i] = v; psbf[i] = v; } converge();
+      VT* ops = vecp(PV_OPS); VT* psfb = vecp(PV_PSFB); VT* psbf = vecp(PV_PSBF);
+      for (int i = tid; i < D; i += TPC) { const VT v = ps[i]; ops[i] = v; psfb[i] = v; psbf[i] = v; } converge();
     }
     s.Vo = s.Vz;
     s.z_is_fwd = 1;
@@ -596,9 +606,9 @@ struct NutsWave {
   __device__ bool leaf() {
     const double e = s.dir * s.eps;
     s.Vz = pending_V();
-    double* p = vecp(PV_P);
-    double* ps = vecp(PV_PS);
-    double* srho = vecp(PV_SRHO);
+    VT* p = vecp(PV_P);
+    VT* ps = vecp(PV_PS);
+    VT* srho = vecp(PV_SRHO);
     s.n_leapfrog++;
     double h;
     if (DENSE) {
@@ -609,7 +619,7 @@ struct NutsWave {
       // is written before the divergence check: a diverging transition ends
       // here and never reads it)
       constexpr bool UNIT = METRIC == METRIC_UNIT;
-      double* edge = s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF);
+      VT* edge = s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF);
       const double hh = 0.5 * e;
       const double a = s.leaf_idx == 0 ? pass_leaf_head<TPC, UNIT, true>(p, g, minv, srho, edge, D, tid, hh)
                                        : pass_leaf_head<TPC, UNIT, false>(p, g, minv, srho, edge, D, tid, hh);
@@ -639,13 +649,13 @@ struct NutsWave {
       else take = uniform() < exp(s.S_logw - lsw_sub);
       if (take) free_slot(s.L_prop[k]);
       else { free_slot(s.s_prop); s.s_prop = s.L_prop[k]; }
-      const double* lb = Lbeg(k);
-      const double* le = Lend(k);
-      const double* lr = Lrho(k);
-      const double* sb = k == 0 ? p : Lbeg(k - 1);
-      const double* lbs = DENSE ? LSbeg(k) : nullptr;
-      const double* les = DENSE ? LSend(k) : nullptr;
-      const double* sbs = DENSE ? (k == 0 ? ps : LSbeg(k - 1)) : nullptr;
+      const VT* lb = Lbeg(k);
+      const VT* le = Lend(k);
+      const VT* lr = Lrho(k);
+      const VT* sb = k == 0 ? p : Lbeg(k - 1);
+      const VT* lbs = DENSE ? LSbeg(k) : nullptr;
+      const VT* les = DENSE ? LSend(k) : nullptr;
+      const VT* sbs = DENSE ? (k == 0 ? ps : LSbeg(k - 1)) : nullptr;
       double acc[6];
       pass_merge<TPC, METRIC>(lb, le, lr, sb, p, srho, minv, lbs, les, sbs, DENSE ? ps : nullptr, D, tid, acc);
       converge();
@@ -674,12 +684,12 @@ struct NutsWave {
     // leap_begin in one pass
     constexpr bool UNIT = METRIC == METRIC_UNIT;
     const int slot = s.s_prop == PROP_LEAF ? alloc_slot() : -1;
-    double* lb = Lbeg(k);
-    double* le = Lend(k);
-    double* lr = Lrho(k);
-    const double* beg = k == 0 ? nullptr : Lbeg(k - 1);
-    double* sq = slot >= 0 ? Sq(slot) : nullptr;
-    double* sg = slot >= 0 ? Sg(slot) : nullptr;
+    VT* lb = Lbeg(k);
+    VT* le = Lend(k);
+    VT* lr = Lrho(k);
+    const VT* beg = k == 0 ? nullptr : Lbeg(k - 1);
+    VT* sq = slot >= 0 ? Sq(slot) : nullptr;
+    VT* sg = slot >= 0 ? Sg(slot) : nullptr;
     const double hh = 0.5 * e;
     if (beg) {
       if (sq) pass_leaf_tail<TPC, UNIT, true, true>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
@@ -714,20 +724,20 @@ struct NutsWave {
     }
     s.log_w = log_sum_exp2(s.log_w, s.S_logw);
     const int fwd = s.z_is_fwd;
-    const double* p = vecp(PV_P);
-    const double* op = vecp(PV_OP);
-    const double* pfb = vecp(PV_PFB);
-    const double* pbf = vecp(PV_PBF);
-    const double* srho = vecp(PV_SRHO);
-    double* rho = vecp(PV_RHO);
-    const double* ps = DENSE ? vecp(PV_PS) : nullptr;
-    const double* ops = DENSE ? vecp(PV_OPS) : nullptr;
-    const double* psfb = DENSE ? vecp(PV_PSFB) : nullptr;
-    const double* psbf = DENSE ? vecp(PV_PSBF) : nullptr;
-    const double* bb = fwd ? op : p;   // p_bck_bck
-    const double* ff = fwd ? p : op;   // p_fwd_fwd
-    const double* bbs = fwd ? ops : ps;
-    const double* ffs = fwd ? ps : ops;
+    const VT* p = vecp(PV_P);
+    const VT* op = vecp(PV_OP);
+    const VT* pfb = vecp(PV_PFB);
+    const VT* pbf = vecp(PV_PBF);
+    const VT* srho = vecp(PV_SRHO);
+    VT* rho = vecp(PV_RHO);
+    const VT* ps = DENSE ? vecp(PV_PS) : nullptr;
+    const VT* ops = DENSE ? vecp(PV_OPS) : nullptr;
+    const VT* psfb = DENSE ? vecp(PV_PSFB) : nullptr;
+    const VT* psbf = DENSE ? vecp(PV_PSBF) : nullptr;
+    const VT* bb = fwd ? op : p;   // p_bck_bck
+    const VT* ff = fwd ? p : op;   // p_fwd_fwd
+    const VT* bbs = fwd ? ops : ps;
+    const VT* ffs = fwd ? ps : ops;
     double acc[6];
     pass_doubling<TPC, METRIC>(fwd, rho, srho, bb, ff, pfb, pbf, minv, bbs, ffs, psfb, psbf, D, tid, acc);
     converge();
@@ -755,33 +765,33 @@ struct NutsWave {
   }
   // var_adaptation::learn_variance / covar_adaptation::learn_covariance;
   // returns true at a window end
-  __device__ bool learn_metric(const double* qs) {
-    double* wm = vecp(PV_WM);
-    double* wm2 = vecp(PV_WM2);
+  __device__ bool learn_metric(const VT* qs) {
+    VT* wm = vecp(PV_WM);
+    VT* wm2 = vecp(PV_WM2);
     if (in_window()) {
       s.welford_n++;
       const double n = (double)s.welford_n;
       if (!DENSE) {
         for (int i = tid; i < D; i += TPC) {
-          const double delta = qs[i] - wm[i];
+          const double delta = (double)qs[i] - (double)wm[i];
           const double m = wm[i] + delta / n;
           wm[i] = m;
           wm2[i] = wm2[i] + (qs[i] - m) * delta;
         } converge();
       } else {
         // welford_covar_estimator: delta = q - m; m += delta/n; m2 += (q - m) deltaᵀ
-        double* delta = vecp(PV_TMP);
+        VT* delta = vecp(PV_TMP);
         for (int i = tid; i < D; i += TPC) {
-          const double dl = qs[i] - wm[i];
+          const double dl = (double)qs[i] - (double)wm[i];
           delta[i] = dl;
-          wm[i] = wm[i] + dl / n;
-          wm2[i] = qs[i] - wm[i];  // (q - m_new), kept in the diag accumulator slot
+          wm[i] = (double)wm[i] + dl / n;
+          wm2[i] = (double)qs[i] - (double)wm[i];  // (q - m_new), kept in the diag accumulator slot
         } converge();
         gsync();
         for (int j = 0; j < D; ++j) {
           const double dj = delta[j];
           double* col = dW + (int64_t)j * D;
-          for (int i = tid; i < D; i += TPC) col[i] = col[i] + wm2[i] * dj; converge();
+          for (int i = tid; i < D; i += TPC) col[i] = col[i] + (double)wm2[i] * dj; converge();
         }
       }
     }
@@ -791,7 +801,7 @@ struct NutsWave {
       int ok = 1;
       if (!DENSE) {
         for (int i = tid; i < D; i += TPC) {
-          const double var = s.welford_n > 1 ? wm2[i] / (n - 1.0) : minv[i];
+          const double var = s.welford_n > 1 ? (double)wm2[i] / (n - 1.0) : (double)minv[i];
           const double v = (n / (n + 5.0)) * var + 1e-3 * (5.0 / (n + 5.0));
           minv[i] = v;
           ok &= isfinite(v) ? 1 : 0;
@@ -846,7 +856,7 @@ struct NutsWave {
         row[5] = s.divergent;
         row[6] = s.slot_H[k];
       } converge();
-      const double* sq = Sq(k);
+      const VT* sq = Sq(k);
       for (int i = tid; i < nsave; i += TPC) row[7 + i] = sq[i]; converge();
       s.draw_idx++;
     }
@@ -949,7 +959,7 @@ __global__ void __launch_bounds__(256) dense_factor_kernel(NutsBuffers B, int* f
 // Stage ChainState in LDS, run, write back.  TPC == 64: one wave per chain, 4
 // chains per block; TPC == 256: one chain per block, each wave with its own
 // identical copy of the state, wave 0 writes it back.
-template <int METRIC, class GradFn, int TPC = NUTS_WAVE>
+template <int METRIC, class GradFn, int TPC = NUTS_WAVE, class VT = double>
 __device__ __forceinline__ void nuts_advance_body(const NutsParams& P, const NutsBuffers& B, const GradFn& grad) {
   __shared__ ChainState sst[4];
   // the wave index is uniform over the wave: keep it (and the chain index and
@@ -970,7 +980,7 @@ __device__ __forceinline__ void nuts_advance_body(const NutsParams& P, const Nut
   // every wave has its copy of the state before wave 0 may write the new one back
   if (TPC > NUTS_WAVE) __syncthreads();
   if (s.phase < PH_PAUSED) {
-    NutsWave<METRIC, GradFn, TPC> wv(P, B, s, grad, c, lane, tid);
+    NutsWave<METRIC, GradFn, TPC, VT> wv(P, B, s, grad, c, lane, tid);
     wv.run();
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

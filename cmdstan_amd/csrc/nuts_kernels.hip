@@ -19,9 +19,9 @@ __global__ void __launch_bounds__(256) advance_external_block_kernel(NutsParams 
   nuts_advance_body<METRIC, ExternalGrad, 256>(P, B, ExternalGrad{});
 }
 
-template <int METRIC, class F>
+template <int METRIC, class F, class VT>
 __global__ void __launch_bounds__(256) advance_fused_kernel(NutsParams P, NutsBuffers B, F f) {
-  nuts_advance_body<METRIC>(P, B, f);
+  nuts_advance_body<METRIC, F, NUTS_WAVE, VT>(P, B, f);
 }
 
 template <class F>
@@ -84,13 +84,23 @@ void launch_dense_factor(const NutsBuffers& B, int* failed, hipStream_t st) {
   hipLaunchKernelGGL(dense_factor_kernel, dim3(blocks_for(B.C)), dim3(256), 0, st, B, failed);
 }
 
+// vec32: the sampler's vectors in FP32 (unit/diag metrics, models with F::vec32)
 template <class F>
-static void fused_for_metric(int metric, const NutsParams& P, const NutsBuffers& B, const F& f, hipStream_t st) {
+static void fused_for_metric(int metric, bool vec32, const NutsParams& P, const NutsBuffers& B, const F& f,
+                             hipStream_t st) {
   const dim3 grid(blocks_for(B.C)), blk(256);
-  if (metric == METRIC_UNIT) hipLaunchKernelGGL((advance_fused_kernel<METRIC_UNIT, F>), grid, blk, 0, st, P, B, f);
+  if constexpr (F::vec32) {
+    if (vec32) {
+      if (metric == METRIC_UNIT)
+        hipLaunchKernelGGL((advance_fused_kernel<METRIC_UNIT, F, float>), grid, blk, 0, st, P, B, f);
+      else hipLaunchKernelGGL((advance_fused_kernel<METRIC_DIAG, F, float>), grid, blk, 0, st, P, B, f);
+      return;
+    }
+  }
+  if (metric == METRIC_UNIT) hipLaunchKernelGGL((advance_fused_kernel<METRIC_UNIT, F, double>), grid, blk, 0, st, P, B, f);
   else if (metric == METRIC_DENSE)
-    hipLaunchKernelGGL((advance_fused_kernel<METRIC_DENSE, F>), grid, blk, 0, st, P, B, f);
-  else hipLaunchKernelGGL((advance_fused_kernel<METRIC_DIAG, F>), grid, blk, 0, st, P, B, f);
+    hipLaunchKernelGGL((advance_fused_kernel<METRIC_DENSE, F, double>), grid, blk, 0, st, P, B, f);
+  else hipLaunchKernelGGL((advance_fused_kernel<METRIC_DIAG, F, double>), grid, blk, 0, st, P, B, f);
 }
 
 #define NUTS_MODEL_SWITCH(kind, md, CALL)                       \
@@ -104,9 +114,16 @@ static void fused_for_metric(int metric, const NutsParams& P, const NutsBuffers&
     default: break;                                             \
   }
 
-void launch_advance_fused(int kind, int metric, const NutsParams& P, const NutsBuffers& B, const ModelDataDev& md,
-                          hipStream_t st) {
-  NUTS_MODEL_SWITCH(kind, md, fused_for_metric(metric, P, B, f, st));
+void launch_advance_fused(int kind, int metric, bool vec32, const NutsParams& P, const NutsBuffers& B,
+                          const ModelDataDev& md, hipStream_t st) {
+  NUTS_MODEL_SWITCH(kind, md, fused_for_metric(metric, vec32, P, B, f, st));
+}
+
+bool model_supports_vec32(int kind) {
+  bool r = false;
+  ModelDataDev md{nullptr, nullptr, 1.0};
+  NUTS_MODEL_SWITCH(kind, md, r = decltype(f)::vec32);
+  return r;
 }
 
 void launch_model_grad(int kind, int C, const int* D, const int64_t* off, const double* Q, double* G, double* V,

@@ -219,7 +219,7 @@ class Model:
         return fit
 
     def transition_replay(self, q0, uniforms, normals, stepsize, metric="diag_e", inv_metric=None, jitter=0.0,
-                          max_depth=10, dataset=0):
+                          max_depth=10, dataset=0, precision=0):
         q0 = _f64(np.atleast_2d(q0))
         nch, D = q0.shape
         u = _f64(np.atleast_2d(uniforms))
@@ -233,8 +233,10 @@ class Model:
         im = _f64(np.broadcast_to(base, (nch, msz)))
         out_q = np.zeros((nch, D))
         info = np.zeros((nch, 9))
-        rc = lib().nuts_transition_replay(self.h, dataset, nch, m, stepsize, jitter, max_depth, _dp(im), _dp(q0),
-                                          _dp(u), u.shape[1], _dp(z), z.shape[1], _dp(out_q), _dp(info))
+        p = {"fp64": 0, "fp32": 1}.get(precision, precision)
+        rc = lib().nuts_transition_replay_precision(self.h, dataset, nch, m, stepsize, jitter, max_depth, _dp(im),
+                                                    _dp(q0), _dp(u), u.shape[1], _dp(z), z.shape[1], p, _dp(out_q),
+                                                    _dp(info))
         if rc != 0:
             raise SamplerError(last_error())
         keys = ["depth", "n_leapfrog", "divergent", "accept_stat", "energy", "lp", "uniforms_used", "normals_used",

@@ -103,7 +103,9 @@ typedef struct nuts_config {
   double init_radius;     /* 2 */
   int fused;              /* -1 auto, 0 lockstep (batched gradient kernels), 1 fused per-chain */
   int precision;          /* batched GLM contraction: 0 FP64 (default), 1 FP32 with X in FP32,
-                             2 FP32 with X stored in BF16; sums, H, U-turn and tree weights stay FP64 */
+                             2 FP32 with X stored in BF16; on the fused per-chain kernel (other
+                             models, unit_e/diag_e) 1 keeps q, p, g, checkpoints and the metric in
+                             FP32 (the FP32 leapfrog); sums, H, U-turn and tree weights stay FP64 */
   int max_saved_params;   /* -1 (default): draws hold the constrained parameters; k >= 0: draws hold
                              the first k unconstrained coordinates only (bounded output for very
                              large batches) */
@@ -203,6 +205,13 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
 int nuts_transition_replay(nuts_model* m, int dataset, int num_chains, int metric, double stepsize, double jitter,
                            int max_depth, const double* inv_metric, const double* q0, const double* u, int64_t n_u,
                            const double* z, int64_t n_z, double* out_q, double* info);
+/* The same with the sampler's vectors at a reduced precision (precision 1: the
+ * FP32 leapfrog of the per-chain kernel; H, tree weights and U-turn sums stay
+ * FP64).  No counterpart in the reference. */
+int nuts_transition_replay_precision(nuts_model* m, int dataset, int num_chains, int metric, double stepsize,
+                                     double jitter, int max_depth, const double* inv_metric, const double* q0,
+                                     const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,
+                                     double* out_q, double* info);
 
 /* ------------------------------------------------------------ summary */
 /* stansummary statistics (src/cmdstan/stansummary_helper.hpp:397-428 get_stats)

@@ -477,3 +477,70 @@ def test_logistic_without_observations_is_the_prior():
     for j in range(1, D + 1):
         s = E.summary(fit.param(f"beta.{j}"))
         assert abs(s["mean"]) < 4 * s["mcse"] and abs(s["sd"] - 2.5) < 0.15
+
+
+FP32_REPLAY = [("hier_normal_nc", "diag_e"), ("eight_schools_cp", "diag_e"), ("gauss", "unit_e"),
+               ("bernoulli", "diag_e"), ("hier_logistic_cp", "diag_e")]
+
+
+@pytest.mark.parametrize("name,metric", FP32_REPLAY)
+def test_fp32_leapfrog_transition_tracks_fp64_oracle(name, metric):
+    """FP32 leapfrog (q, p, g, checkpoints and metric stored in FP32; FP64
+    arithmetic, H, tree weights and U-turn sums): from the same FP32-exact start
+    and metric with the same injected draws, the transition builds the FP64
+    oracle's tree in nearly every chain, and there the selected draw agrees to
+    1e-4 of the coordinate scale and the energy to 1e-5 relative.  A wrong
+    term in the FP32 passes moves every trajectory by far more than that."""
+    data = CASES[name]
+    gm, om = ca.Model(name, data), O.Model(name, data)
+    D = gm.dim
+    rng = np.random.default_rng(29)
+    nch = 32
+    q0 = (rng.standard_normal((nch, D)) * 0.5).astype(np.float32).astype(np.float64)
+    u = rng.random((nch, 4096))
+    z = rng.standard_normal((nch, D))
+    minv = rng.uniform(0.3, 2.0, D) if metric == "diag_e" else np.ones(D)
+    minv = minv.astype(np.float32).astype(np.float64)
+    agree = total = 0
+    worst = 0.0
+    for eps in [0.02, 0.1, 0.3]:
+        qg, ig = gm.transition_replay(q0, u, z, eps, metric=metric, inv_metric=minv, max_depth=8, precision="fp32")
+        for c in range(nch):
+            qo, io = om.transition_replay(q0[c], u[c], z[c], eps, metric=O.METRICS[metric], minv=minv, max_depth=8)
+            total += 1
+            if any(ig[c][k] != io[k] for k in ("depth", "n_leapfrog", "divergent", "uniforms_used")):
+                continue
+            scale = max(1.0, float(np.max(np.abs(qo))))
+            err = float(np.max(np.abs(qg[c] - qo))) / scale
+            if err > 1e-4:
+                continue  # a knife-edge multinomial choice picked another leaf
+            agree += 1
+            worst = max(worst, err)
+            assert ig[c]["energy"] == pytest.approx(io["energy"], rel=1e-5, abs=1e-5), (name, eps, c)
+    assert agree >= 0.9 * total, (name, agree, total)
+    assert worst > 0  # the FP32 path really rounds
+
+
+def test_fp32_leapfrog_batched_hier_normal_matches_oracle_posterior():
+    """Batched uneven hierarchical normal models with the FP32 leapfrog: means
+    within 4 combined MCSE of the FP64 oracle's, R-hat < 1.02."""
+    datasets = [hier_normal(seed=s, J=J) for s, J in [(1, 8), (2, 30), (3, 75)]]
+    gm = ca.Model("hier_normal_nc", datasets)
+    fit = gm.sample(num_chains=8, seed=23, delta=0.9, precision="fp32")
+    assert np.all(fit.status == 0)
+    for m, data in enumerate(datasets):
+        r = O.Model("hier_normal_nc", data).run(seed=21, num_chains=4, delta=0.9)
+        for name, col in [("mu", 7), ("tau", 8)]:
+            xg = fit.param(name, dataset=m)
+            xo = r["draws"][:, :, col].T
+            sg, so = E.summary(xg), E.summary(xo)
+            assert sg["rhat"] < 1.02, (m, name, sg["rhat"])
+            assert abs(sg["mean"] - so["mean"]) < 4 * np.hypot(sg["mcse"], so["mcse"]), (m, name)
+
+
+def test_fp32_leapfrog_rejects_dense_and_bf16():
+    gm = ca.Model("gauss", CASES["gauss"])
+    with pytest.raises(ca.SamplerError):
+        gm.sample(num_chains=2, num_warmup=10, num_samples=10, metric="dense_e", precision="fp32", fused=1)
+    with pytest.raises(ca.SamplerError):
+        gm.sample(num_chains=2, num_warmup=10, num_samples=10, precision="bf16", fused=1)

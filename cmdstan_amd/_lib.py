@@ -37,7 +37,7 @@ class nuts_output(C.Structure):
         ("pilot_ess_per_grad_plain", C.c_double), ("pilot_ess_per_grad_flow", C.c_double),
         ("pilot_ess_per_sec_plain", C.c_double), ("pilot_ess_per_sec_flow", C.c_double),
         ("pilot_rhat_plain", C.c_double), ("pilot_rhat_flow", C.c_double),
-        ("tuned_path", C.c_int), ("tuned_blocks_per_cu", C.c_int),
+        ("tuned_path", C.c_int), ("tuned_blocks_per_cu", C.c_int), ("diagnostics", DP),
     ]
 
 

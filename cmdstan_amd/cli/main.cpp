@@ -359,6 +359,14 @@ int run(int argc, char** argv) {
   std::memset(&out, 0, sizeof out);
   out.draws = draws.data(); out.stepsize = step.data(); out.inv_metric = imet.data();
   out.n_leapfrog = nlf.data(); out.grad_evals = nge.data(); out.status = status.data();
+  // diagnostic_file: q, p, g of every saved draw (mcmc_writer::write_diagnostic_params)
+  const std::string diag_file = val(R, {"output", "diagnostic_file"});
+  const int dcols = 7 + 3 * D;
+  std::vector<double> diag;
+  if (!diag_file.empty()) {
+    diag.resize((size_t)C * rows * dcols);
+    out.diagnostics = diag.data();
+  }
   std::cout << "Running " << C << " chain(s) of NUTS on the GPU (" << nuts_version() << ")" << std::endl;
   // init=<file>: per-chain files {name}_{id+i}{ext} when the first exists, else
   // the named file for every chain (command_helper.hpp:144-245)
@@ -453,6 +461,26 @@ int run(int argc, char** argv) {
         for (int i = 0; i < D; ++i) mf << (i ? ", " : "") << m[i];
       }
       mf << "]\n}\n";
+    }
+  }
+  if (!diag_file.empty()) {
+    std::vector<std::string> un;
+    for (int i = 0; i < D; ++i) { nuts_model_unconstrained_name(model, 0, i, buf, sizeof buf); un.push_back(buf); }
+    const auto dfiles = make_filenames(diag_file, "", ".csv", C, cfg.id);
+    for (int k = 0; k < C; ++k) {
+      std::ofstream f(dfiles[k]);
+      if (!f) throw std::invalid_argument("Can't open diagnostic file " + dfiles[k]);
+      if (sig_figs > -1) f << std::setprecision(sig_figs);
+      std::istringstream ct(cfgtxt.str());
+      std::string line;
+      while (std::getline(ct, line)) f << "# " << line << "\n";
+      f << "lp__,accept_stat__,stepsize__,treedepth__,n_leapfrog__,divergent__,energy__";
+      for (auto& n : un) f << "," << n;
+      for (auto& n : un) f << ",p_" << n;
+      for (auto& n : un) f << ",g_" << n;
+      f << "\n";
+      const double* d = diag.data() + (size_t)k * rows * dcols;
+      for (int r = 0; r < rows; ++r) fmt_row(f, d + (size_t)r * dcols, dcols);
     }
   }
   std::cout << "Elapsed Time: " << out.warmup_seconds << " seconds (Warm-up), " << out.sampling_seconds

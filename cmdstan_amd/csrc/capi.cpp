@@ -318,6 +318,7 @@ struct RunSpec {
   bool constrain = true;
   FlowNet* flow = nullptr;                      // sample z with x = T(z) (single dataset, lockstep)
   const double* stepsize_per_chain = nullptr;   // [C] resume with these step sizes (no adaptation)
+  bool diagnostics = false;                     // record q, p, g of every saved draw
 };
 
 struct RunResult {
@@ -334,6 +335,7 @@ struct RunResult {
   int tuned_path = -1, tuned_bpc = 0;
   std::vector<double> z_last;  // [C][D] unconstrained q of each chain's last saved row (before any flow map)
   int rows = 0;
+  std::vector<double> diag;    // diagnostics: chains back to back, rows x (7 + 3 D)
 };
 
 int output_rows(const nuts_config& c) {
@@ -477,7 +479,8 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   const int maxd = cfg.max_depth;
   const int nslots = maxd + 2;
   const bool dense = cfg.metric == NUTS_METRIC_DENSE_E;
-  const int NV = PV_FIXED + 3 * maxd + 2 * nslots + (dense ? 2 * maxd : 0);
+  const int NV = PV_FIXED + 3 * maxd + 2 * nslots + (dense ? 2 * maxd : 0) + (R.diagnostics ? nslots : 0);
+  if (R.diagnostics && R.flow) throw std::invalid_argument("diagnostic rows are not available with the flow");
   int64_t mtot = 0;
   if (dense) {
     for (int c = 0; c < C; ++c) {
@@ -502,6 +505,19 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   DevBuf<double> Q(tot), G(tot), V(C), pool((size_t)tot * NV), minv(tot), draws(std::max<int64_t>(dtot, 1));
   DevBuf<ChainState> dst(C);
   DevBuf<double> uinit, du, dz;
+  std::vector<int64_t> diag_off(C);
+  int64_t diag_tot = 0;
+  for (int c = 0; c < C; ++c) {
+    diag_off[c] = diag_tot;
+    diag_tot += (int64_t)rows * (7 + 3 * D[c]);
+  }
+  DevBuf<double> ddiag;
+  DevBuf<int64_t> ddiag_off;
+  if (R.diagnostics) {
+    ddiag.alloc(std::max<int64_t>(diag_tot, 1));
+    ddiag_off.alloc(C);
+    ddiag_off.upload(diag_off, st);
+  }
   dD.upload(D, st); dDp.upload(Dp, st); doffb.upload(off, st); dmoff.upload(moff, st); ddoff.upload(doff, st);
   ddsoff.upload(dsoff, st); dooff.upload(ooff, st); duid.upload(uid, st);
   HIPC(hipMemsetAsync(Q.p, 0, tot * sizeof(double), st));
@@ -588,6 +604,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   P.nslots = nslots; P.maxd = maxd; P.NV = NV;
   P.fused_budget = 256;
   P.save_params = msp;
+  P.diag = R.diagnostics ? 1 : 0;
 
   NutsBuffers B;
   std::memset(&B, 0, sizeof(B));
@@ -598,6 +615,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   B.inj_u = R.inj_u ? du.p : nullptr; B.inj_z = R.inj_z ? dz.p : nullptr;
   B.n_inj_u = R.n_u; B.n_inj_z = R.n_z;
   B.dM = dMb.p; B.dL = dLb.p; B.dW = dWb.p;
+  B.diag = ddiag.p; B.diag_off = ddiag_off.p;
   if (dense) {
     DevBuf<int> nfail(1);
     HIPC(hipMemsetAsync(nfail.p, 0, sizeof(int), st));
@@ -935,6 +953,10 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     HIPC(hipMemcpyAsync(res.draws.data(), draws.p, dtot * sizeof(double), hipMemcpyDeviceToHost, st));
     res.draw_off = doff;
     for (int c = 0; c < C; ++c) res.cols.push_back(7 + nsave_of(D[c]));
+  }
+  if (R.diagnostics) {
+    res.diag.resize(diag_tot);
+    HIPC(hipMemcpyAsync(res.diag.data(), ddiag.p, diag_tot * sizeof(double), hipMemcpyDeviceToHost, st));
   }
   HIPC(hipStreamSynchronize(st));
   HIPC(hipGetLastError());
@@ -1373,6 +1395,8 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
     R.cfg = *cfg;
     R.init = init;
     R.inv_metric = inv_metric;
+    R.diagnostics = out && out->diagnostics;
+    if (R.diagnostics && cfg->flow) throw std::invalid_argument("diagnostic rows are not available with the flow");
     if (init || inv_metric) {
       // per-chain arrays index with a uniform stride: require one dimension
       for (auto& d : m->impl.ds)
@@ -1392,6 +1416,7 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
         if (out->status) out->status[c] = s.phase == PH_DONE ? 0 : (s.failure ? s.failure : -1);
       }
       if (out->inv_metric) std::memcpy(out->inv_metric, r.minv.data(), r.minv.size() * sizeof(double));
+      if (out->diagnostics) std::memcpy(out->diagnostics, r.diag.data(), r.diag.size() * sizeof(double));
       (void)moff;
       out->warmup_seconds = r.warm_s;
       out->sampling_seconds = r.samp_s;

@@ -106,14 +106,19 @@ __device__ __forceinline__ void pass_leaf_tail(VT* __restrict__ p, VT* __restric
                                                const VT* __restrict__ minv, const VT* __restrict__ rho,
                                                const VT* __restrict__ beg, VT* __restrict__ lbeg,
                                                VT* __restrict__ lend, VT* __restrict__ lrho, VT* __restrict__ sq,
-                                               VT* __restrict__ sg, int D, int tid, double h, double e) {
+                                               VT* __restrict__ sg, VT* __restrict__ sp, int D, int tid, double h,
+                                               double e) {
 #pragma unroll 4
   for (int i = tid; i < D; i += TPC) {
     const VT po = p[i], qo = q[i], gi = g[i];
     lbeg[i] = BEG ? beg[i] : po;
     lend[i] = po;
     lrho[i] = rho[i];
-    if (SLOT) { sq[i] = qo; sg[i] = gi; }
+    if (SLOT) {
+      sq[i] = qo;
+      sg[i] = gi;
+      if (sp) sp[i] = po;
+    }
     const VT pn = (double)po - h * (double)gi;
     p[i] = pn;
     q[i] = (double)qo + e * ((UNIT ? 1.0 : (double)minv[i]) * (double)pn);
@@ -278,6 +283,8 @@ struct NutsWave {
   __device__ VT* Sg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + P.nslots + k); }
   __device__ VT* LSbeg(int k) const { return vecp(PV_FIXED + 3 * P.maxd + 2 * P.nslots + k); }
   __device__ VT* LSend(int k) const { return vecp(PV_FIXED + 4 * P.maxd + 2 * P.nslots + k); }
+  // momentum of a proposal slot (kept only for the diagnostic rows)
+  __device__ VT* Sp(int k) const { return vecp(PV_FIXED + (DENSE ? 5 : 3) * P.maxd + 2 * P.nslots + k); }
 
   // ------------------------------------------------------------- metric
   __device__ double mi(int i) const { return METRIC == METRIC_UNIT ? 1.0 : (double)minv[i]; }
@@ -432,6 +439,7 @@ struct NutsWave {
     const int k = alloc_slot();
     copy(Sq(k), q);
     copy(Sg(k), g);
+    if (P.diag) copy(Sp(k), vecp(PV_P));
     s.slot_V[k] = s.Vz;
     s.slot_H[k] = s.leaf_h;
     return k;
@@ -555,6 +563,8 @@ struct NutsWave {
     copy(g, Sg(k));
     s.Vz = s.slot_V[k];
     sample_p(p, ps);
+    // the draw stays at z0 (with the new momentum) unless a leaf replaces it
+    if (P.diag) copy(Sp(k), p);
     VT* oq = vecp(PV_OQ); VT* op = vecp(PV_OP); VT* og = vecp(PV_OG);
     VT* pfb = vecp(PV_PFB); VT* pbf = vecp(PV_PBF); VT* rho = vecp(PV_RHO);
     for (int i = tid; i < D; i += TPC) {
@@ -690,13 +700,14 @@ struct NutsWave {
     const VT* beg = k == 0 ? nullptr : Lbeg(k - 1);
     VT* sq = slot >= 0 ? Sq(slot) : nullptr;
     VT* sg = slot >= 0 ? Sg(slot) : nullptr;
+    VT* spp = slot >= 0 && P.diag ? Sp(slot) : nullptr;
     const double hh = 0.5 * e;
     if (beg) {
-      if (sq) pass_leaf_tail<TPC, UNIT, true, true>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
-      else pass_leaf_tail<TPC, UNIT, true, false>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
+      if (sq) pass_leaf_tail<TPC, UNIT, true, true>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, spp, D, tid, hh, e);
+      else pass_leaf_tail<TPC, UNIT, true, false>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, spp, D, tid, hh, e);
     } else {
-      if (sq) pass_leaf_tail<TPC, UNIT, false, true>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
-      else pass_leaf_tail<TPC, UNIT, false, false>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, D, tid, hh, e);
+      if (sq) pass_leaf_tail<TPC, UNIT, false, true>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, spp, D, tid, hh, e);
+      else pass_leaf_tail<TPC, UNIT, false, false>(p, q, g, minv, srho, beg, lb, le, lr, sq, sg, spp, D, tid, hh, e);
     }
     converge();
     if (slot >= 0) {
@@ -858,6 +869,20 @@ struct NutsWave {
       } converge();
       const VT* sq = Sq(k);
       for (int i = tid; i < nsave; i += TPC) row[7 + i] = sq[i]; converge();
+      if (P.diag) {
+        // diagnostic_file row: sampler columns, q, p, g of the draw
+        double* dr = B.diag + B.diag_off[c] + (int64_t)s.draw_idx * (7 + 3 * D);
+        if (tid == 0) {
+          for (int j = 0; j < 7; ++j) dr[j] = row[j];
+        } converge();
+        const VT* sp = Sp(k);
+        const VT* sg = Sg(k);
+        for (int i = tid; i < D; i += TPC) {
+          dr[7 + i] = sq[i];
+          dr[7 + D + i] = sp[i];
+          dr[7 + 2 * D + i] = sg[i];
+        } converge();
+      }
       s.draw_idx++;
     }
     if (s.adapt_flag) {

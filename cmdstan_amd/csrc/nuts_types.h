@@ -78,7 +78,8 @@ struct NutsParams {
   uint32_t w_counter0, w_size0, w_next0, metric_adapt;
   uint64_t seed;
   int nslots, maxd, NV, fused_budget;
-  int save_params, pad_p;  // -1: all D coordinates in each draw row; else the first save_params
+  int save_params;  // -1: all D coordinates in each draw row; else the first save_params
+  int diag;         // 1: record the diagnostic rows (q, p, g of each saved draw)
 };
 
 // Device pointers (kernel argument by value).
@@ -107,6 +108,10 @@ struct NutsBuffers {
   double* dM;               // inverse metric
   double* dL;               // its lower Cholesky factor
   double* dW;               // Welford covariance accumulator m2
+  // diagnostic rows (P.diag): [C] blocks of rows x (7 + 3 D_c) at diag_off[c]:
+  // the sampler columns, then q, p, g of the draw
+  double* diag;
+  const int64_t* diag_off;
 };
 
 // Vector ids inside a chain's pool block.
@@ -115,7 +120,7 @@ enum PoolVec : int {
   // dense_e companions: Minv times the momentum of the same role, and scratch
   PV_PS, PV_OPS, PV_PSFB, PV_PSBF, PV_TMP,
   PV_FIXED  // then L_beg[maxd], L_end[maxd], L_rho[maxd], slot_q[nslots], slot_g[nslots],
-            // and for dense_e LS_beg[maxd], LS_end[maxd]
+            // for dense_e LS_beg[maxd], LS_end[maxd], and with diagnostics slot_p[nslots]
 };
 
 }  // namespace nutsgpu

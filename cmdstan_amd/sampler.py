@@ -136,7 +136,7 @@ class Model:
                window=25, max_depth=10, metric="diag_e", stepsize=1.0, stepsize_jitter=0.0, id=1, seed=0,
                init=2.0, inv_metric=None, fused=-1, precision=0, max_saved_params=-1, flow=False, flow_layers=8,
                flow_hidden=128, flow_train_steps=1000, flow_batch=256, flow_lr=1e-3, flow_warmup=300,
-               flow_pilot=100, refresh=0):
+               flow_pilot=100, refresh=0, diagnostics=False):
         cfg = nuts_config()
         lib().nuts_config_default(C.byref(cfg))
         cfg.num_samples, cfg.num_warmup, cfg.save_warmup, cfg.thin = num_samples, num_warmup, int(save_warmup), thin
@@ -187,6 +187,9 @@ class Model:
         out.n_leapfrog = nlf.ctypes.data_as(I64P)
         out.grad_evals = nge.ctypes.data_as(I64P)
         out.status = status.ctypes.data_as(C.POINTER(C.c_int))
+        dcols = [7 + 3 * self.dims[m] for m in ds_of_chain]
+        diag = np.zeros(sum(rows * c for c in dcols)) if diagnostics else None
+        out.diagnostics = _dp(diag)
         rc = lib().nuts_sample(self.h, C.byref(cfg), _dp(init_arr), _dp(im), C.byref(out))
         err = last_error() if rc != 0 else ""
         per_chain, o, mo, metrics = [], 0, 0, []
@@ -201,7 +204,14 @@ class Model:
             else:
                 metrics.append(imet[mo:mo + D])
                 mo += D
-        fit = Fit(draws=per_chain, stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
+        diag_rows = None
+        if diagnostics:
+            diag_rows, o = [], 0
+            for k in range(C_tot):
+                n = rows * dcols[k]
+                diag_rows.append(diag[o:o + n].reshape(rows, dcols[k]))
+                o += n
+        fit = Fit(draws=per_chain, diagnostics=diag_rows, stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
                   status=status, warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds,
                   leaf_steps=out.leaf_steps, grad_seconds_per_launch=out.grad_seconds_per_launch,
                   grad_launches_timed=out.grad_launches_timed,

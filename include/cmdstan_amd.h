@@ -180,6 +180,11 @@ typedef struct nuts_output {
      kernels; -1 when not applicable */
   int tuned_path;
   int tuned_blocks_per_cu;
+  /* diagnostic_file rows (mcmc_writer::write_diagnostic_params): when not NULL,
+   * chains back to back, each rows x (7 + 3 D): the seven sampler columns, then
+   * the unconstrained draw q, its momentum p and the potential gradient
+   * g = -grad log p (columns named <name>, p_<name>, g_<name>).  Not with flow. */
+  double* diagnostics;
 } nuts_output;
 
 /* model_base::transform_inits (the `init=` file path of command_helper.hpp:144-245):

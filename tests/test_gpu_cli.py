@@ -55,3 +55,21 @@ def test_per_chain_init_files_and_bad_init(tmp_path):
     assert r.returncode != 0 and "theta" in r.stdout + r.stderr
     r = _run(["sample", "num_chains=2", "data", f"file={DATA}", "init=missing.json"], tmp_path)
     assert r.returncode != 0 and "Can't open" in r.stdout + r.stderr
+
+
+def test_diagnostic_file_per_chain(tmp_path):
+    """multi_chain_test.cpp:14-80: id=10, two chains, diagnostic_file=<base>_diag.csv
+    gives <base>_diag_10.csv and <base>_diag_11.csv; each row holds the sample
+    row's sampler columns, then q, p_<name>, g_<name>."""
+    r = _run(["id=10", "sample", "num_warmup=200", "num_samples=50", "num_chains=2", "data", f"file={DATA}",
+              "random", "seed=1234", "output", "file=m.csv", "diagnostic_file=m_diag.csv"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for k in (10, 11):
+        cols, draws, _ = load_stan_csv(str(tmp_path / f"m_{k}.csv"))
+        dcols, diag, _ = load_stan_csv(str(tmp_path / f"m_diag_{k}.csv"))
+        assert dcols == ["lp__", "accept_stat__", "stepsize__", "treedepth__", "n_leapfrog__", "divergent__",
+                         "energy__", "theta", "p_theta", "g_theta"]
+        assert diag.shape == (50, 10)
+        assert np.allclose(diag[:, :7], draws[:, :7], rtol=1e-5)
+        # theta = inv_logit(q)
+        assert np.allclose(1 / (1 + np.exp(-diag[:, 7])), draws[:, 7], rtol=1e-5)

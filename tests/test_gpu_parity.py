@@ -544,3 +544,40 @@ def test_fp32_leapfrog_rejects_dense_and_bf16():
         gm.sample(num_chains=2, num_warmup=10, num_samples=10, metric="dense_e", precision="fp32", fused=1)
     with pytest.raises(ca.SamplerError):
         gm.sample(num_chains=2, num_warmup=10, num_samples=10, precision="bf16", fused=1)
+
+
+@pytest.mark.parametrize("metric,precision", [("diag_e", "fp64"), ("unit_e", "fp64"), ("dense_e", "fp64"),
+                                              ("diag_e", "fp32")])
+def test_diagnostic_rows_hold_q_p_g_of_the_draw(metric, precision):
+    """Diagnostic rows (mcmc_writer::write_diagnostic_params): q is the draw's
+    unconstrained value (tau = exp(q_tau) in the sample row), g the potential
+    gradient at q (log_prob_grad, Jacobian included), lp__ the log density
+    there, and p the momentum the energy column was computed with:
+    energy__ = -lp__ + 1/2 p' Minv p, also for draws that stayed at z0 with the
+    transition's fresh momentum."""
+    gm = ca.Model("eight_schools_cp", EIGHT)
+    D = gm.dim
+    fit = gm.sample(num_chains=4, num_warmup=200, num_samples=100, seed=3, metric=metric, precision=precision,
+                    fused=1, diagnostics=True)
+    assert np.all(fit.status == 0)
+    for k in range(4):
+        dr, rows = fit.diagnostics[k], fit.draws[k]
+        assert dr.shape == (100, 7 + 3 * D)
+        assert np.array_equal(dr[:, :7], rows[:, :7])
+        q, p, g = dr[:, 7:7 + D], dr[:, 7 + D:7 + 2 * D], dr[:, 7 + 2 * D:]
+        j_tau = fit.column_names[0].index("tau")
+        assert np.allclose(np.exp(q[:, 1]), rows[:, j_tau], rtol=1e-12)
+        lp, grad = gm.log_prob_grad(q)
+        assert np.allclose(lp, dr[:, 0], rtol=1e-12, atol=1e-10)
+        assert np.allclose(-grad, g, rtol=1e-6 if precision == "fp32" else 1e-10, atol=1e-6)
+        M = fit.inv_metric[k]
+        kin = 0.5 * (np.einsum("ri,ij,rj->r", p, M, p) if metric == "dense_e" else
+                     np.sum(p * p * (M if metric == "diag_e" else 1.0), axis=1))
+        assert np.allclose(dr[:, 6], -dr[:, 0] + kin, rtol=1e-10, atol=1e-9)
+    assert np.all(np.isfinite(fit.diagnostics[0]))
+
+
+def test_diagnostic_rows_rejected_with_flow():
+    gm = ca.Model("gauss", CASES["gauss"])
+    with pytest.raises(ca.SamplerError):
+        gm.sample(num_chains=2, num_warmup=20, num_samples=10, flow="force", diagnostics=True)

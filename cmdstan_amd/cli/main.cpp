@@ -165,6 +165,51 @@ void print_tree(const Node* n, int depth, std::ostream& o, const std::string& pr
   }
 }
 
+// JSON form of the argument tree (write_config.hpp:31-47 with the structured
+// writer): groups are objects, a list is {"value": choice, choice: {...}},
+// values are numbers, booleans or strings.
+std::string json_value(const std::string& v) {
+  if (v == "true" || v == "false") return v;
+  char* e = nullptr;
+  if (!v.empty() && v.find_first_not_of("0123456789+-.eE") == std::string::npos) {
+    std::strtod(v.c_str(), &e);
+    if (*e == '\0') return v;
+  }
+  std::string o = "\"";
+  for (char ch : v) {
+    if (ch == '"' || ch == '\\') o += '\\';
+    o += ch;
+  }
+  return o + "\"";
+}
+
+void print_json_members(const Node* n, std::ostream& o, int depth, bool& first) {
+  const std::string ind(2 * depth, ' ');
+  for (auto& k : n->kids) {
+    if (n->kind == Node::LIST) continue;  // a list prints its chosen member itself
+    o << (first ? "" : ",\n") << ind << "\"" << k->name << "\": ";
+    first = false;
+    if (k->kind == Node::VALUE) {
+      o << json_value(k->value);
+    } else if (k->kind == Node::GROUP) {
+      o << "{\n";
+      bool f = true;
+      print_json_members(k.get(), o, depth + 1, f);
+      o << "\n" << ind << "}";
+    } else {
+      o << "{\n" << ind << "  \"value\": " << json_value(k->value);
+      const Node* chosen = k->kid(k->value);
+      if (chosen && chosen->kind == Node::GROUP) {
+        o << ",\n" << ind << "  \"" << chosen->name << "\": {\n";
+        bool f = true;
+        print_json_members(chosen, o, depth + 2, f);
+        o << "\n" << ind << "  }";
+      }
+      o << "\n" << ind << "}";
+    }
+  }
+}
+
 const Node* find(const Node* n, const std::vector<std::string>& path) {
   for (auto& p : path) {
     n = n->kid(p);
@@ -318,6 +363,23 @@ int run(int argc, char** argv) {
   std::ostringstream cfgtxt;
   print_tree(R, 0, cfgtxt, "");
   std::cout << cfgtxt.str() << std::endl;
+  if (to_bool(val(R, {"output", "save_cmdstan_config"}), "save_cmdstan_config")) {
+    // <output basename>_config.json (command.hpp:279-285)
+    std::string base = out_file;
+    const size_t dot = base.find_last_of('.'), sep = base.find_last_of('/');
+    if (dot != std::string::npos && (sep == std::string::npos || dot > sep) && dot > 0) base = base.substr(0, dot);
+    std::ofstream jf(base + "_config.json");
+    if (!jf) throw std::invalid_argument("Can't open config file " + base + "_config.json");
+    std::time_t now = std::time(nullptr);
+    char tb[64];
+    std::strftime(tb, sizeof tb, "%Y-%m-%d %H:%M:%S UTC", std::gmtime(&now));
+    jf << "{\n  \"stan_major_version\": 2,\n  \"stan_minor_version\": 36,\n  \"stan_patch_version\": 0,\n"
+       << "  \"model_name\": " << json_value(family + "_model") << ",\n  \"start_datetime\": " << json_value(tb)
+       << ",\n";
+    bool first = true;
+    print_json_members(R, jf, 1, first);
+    jf << ",\n  \"mpi_enabled\": false,\n  \"engine\": " << json_value(nuts_version()) << "\n}\n";
+  }
 
   nuts_data* data = nuts_data_create();
   if (!data_file.empty() && nuts_data_parse_json(data, read_file(data_file).c_str()) != NUTS_OK)

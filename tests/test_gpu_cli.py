@@ -73,3 +73,20 @@ def test_diagnostic_file_per_chain(tmp_path):
         assert np.allclose(diag[:, :7], draws[:, :7], rtol=1e-5)
         # theta = inv_logit(q)
         assert np.allclose(1 / (1 + np.exp(-diag[:, 7])), draws[:, 7], rtol=1e-5)
+
+
+def test_save_cmdstan_config_json(tmp_path):
+    """config_json_test.cpp: save_cmdstan_config=1 writes <output base>_config.json,
+    valid JSON holding the argument tree; without it no file is written."""
+    r = _run(["sample", "num_samples=20", "num_warmup=20", "data", f"file={DATA}", "output", "file=o.csv",
+              "save_cmdstan_config=1"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg = json.loads((tmp_path / "o_config.json").read_text())
+    assert cfg["method"]["value"] == "sample"
+    s = cfg["method"]["sample"]
+    assert s["num_samples"] == 20 and s["num_warmup"] == 20
+    assert s["algorithm"]["value"] == "hmc" and s["algorithm"]["hmc"]["engine"]["nuts"]["max_depth"] == 10
+    assert cfg["data"]["file"] == DATA and cfg["output"]["file"] == "o.csv"
+    assert cfg["model_name"] == "bernoulli_model" and cfg["stan_major_version"] == 2
+    r = _run(["sample", "num_samples=20", "num_warmup=20", "data", f"file={DATA}", "output", "file=p.csv"], tmp_path)
+    assert r.returncode == 0 and not (tmp_path / "p_config.json").exists()

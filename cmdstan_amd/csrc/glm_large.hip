@@ -22,6 +22,28 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float gl_f32(float x) { return x; }
 __device__ __forceinline__ float gl_f32(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
 
+// eight consecutive elements from a 16-byte aligned address: one 16-byte load
+// (BF16) or two (FP32), widened to FP32
+__device__ __forceinline__ void gl_load8(const float* p, float (&v)[8]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  const float4 b = *reinterpret_cast<const float4*>(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void gl_load8(const unsigned short* p, float (&v)[8]) {
+  const uint4 a = *reinterpret_cast<const uint4*>(p);
+  const unsigned w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[2 * i] = __uint_as_float(w[i] << 16);
+    v[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+  }
+}
+template <typename T>
+__device__ __forceinline__ bool gl_aligned16(const T* p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
 struct GlTile {
   int M, N, K;
   int64_t sam, sak, sbk, sbn;
@@ -38,22 +60,49 @@ __device__ __forceinline__ void gl_tile(const AT* __restrict__ A, const BT* __re
   const bool a_k = t.sak == 1;
   const bool b_n = t.sbn == 1;
   for (int k0 = kb; k0 < ke; k0 += GL_BK) {
-    // A tile 128 x 16: 8 elements per thread
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int mm, kk;
-      if (a_k) { mm = (tid >> 1); kk = (tid & 1) * 8 + e; }
-      else { kk = tid >> 4; mm = (tid & 15) * 8 + e; }
+    // A tile 128 x 16: 8 elements per thread, contiguous in memory along k
+    // (a_k) or along m; one vector load when all 8 are in range and aligned
+    {
+      const int mm = a_k ? (tid >> 1) : (tid & 15) * 8, kk = a_k ? (tid & 1) * 8 : (tid >> 4);
       const int m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < t.M && k < ke) ? gl_f32(A[(int64_t)m * t.sam + (int64_t)k * t.sak]) : 0.0f;
-    }
+      const AT* src = A + (int64_t)m * t.sam + (int64_t)k * t.sak;
+      const bool full = a_k ? (m < t.M && k + 7 < ke) : (m + 7 < t.M && k < ke);
+      float v[8];
+      if (full && gl_aligned16(src)) {
+        gl_load8(src, v);
+      } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int nn, kk;
-      if (b_n) { kk = tid >> 4; nn = (tid & 15) * 8 + e; }
-      else { nn = (tid >> 1); kk = (tid & 1) * 8 + e; }
+        for (int e = 0; e < 8; ++e) {
+          const int me = a_k ? m : m + e, ke_ = a_k ? k + e : k;
+          v[e] = (me < t.M && ke_ < ke) ? gl_f32(A[(int64_t)me * t.sam + (int64_t)ke_ * t.sak]) : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        if (a_k) As[kk + e][mm] = v[e];
+        else As[kk][mm + e] = v[e];
+      }
+    }
+    {
+      const int nn = b_n ? (tid & 15) * 8 : (tid >> 1), kk = b_n ? (tid >> 4) : (tid & 1) * 8;
       const int n = n0 + nn, k = k0 + kk;
-      Bs[kk][nn] = (n < t.N && k < ke) ? gl_f32(B[(int64_t)k * t.sbk + (int64_t)n * t.sbn]) : 0.0f;
+      const BT* src = B + (int64_t)k * t.sbk + (int64_t)n * t.sbn;
+      const bool full = b_n ? (n + 7 < t.N && k < ke) : (n < t.N && k + 7 < ke);
+      float v[8];
+      if (full && gl_aligned16(src)) {
+        gl_load8(src, v);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int ne = b_n ? n + e : n, ke_ = b_n ? k : k + e;
+          v[e] = (ne < t.N && ke_ < ke) ? gl_f32(B[(int64_t)ke_ * t.sbk + (int64_t)ne * t.sbn]) : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        if (b_n) Bs[kk][nn + e] = v[e];
+        else Bs[kk + e][nn] = v[e];
+      }
     }
     __syncthreads();
 #pragma unroll

@@ -5,7 +5,7 @@
 //           R = sigmoid(Eta) - y stored in FP32, lp terms summed per chain in FP64
 //   pass 2  G^T = R^T . X       (active chains x D), split over row ranges
 //   reduce  FP64 sum of the row-range partials + prior; V = -(lp - prior)
-// Both passes use one 128x128x16 tile kernel (4 waves of 64x64) with the
+// Both passes use one 128x128x32 tile kernel (4 waves of 64x64) with the
 // operand strides as parameters; BF16 elements are widened to FP32 on their
 // way into LDS.
 #include "device_common.h"
@@ -17,7 +17,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 #define GL_BM 128
 #define GL_BN 128
-#define GL_BK 16
+#define GL_BK 32
 
 __device__ __forceinline__ float gl_f32(float x) { return x; }
 __device__ __forceinline__ float gl_f32(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
@@ -60,10 +60,12 @@ __device__ __forceinline__ void gl_tile(const AT* __restrict__ A, const BT* __re
   const bool a_k = t.sak == 1;
   const bool b_n = t.sbn == 1;
   for (int k0 = kb; k0 < ke; k0 += GL_BK) {
-    // A tile 128 x 16: 8 elements per thread, contiguous in memory along k
-    // (a_k) or along m; one vector load when all 8 are in range and aligned
-    {
-      const int mm = a_k ? (tid >> 1) : (tid & 15) * 8, kk = a_k ? (tid & 1) * 8 : (tid >> 4);
+    // A tile 128 x GL_BK in 128 x 16 slabs: 8 elements per thread per slab,
+    // contiguous in memory along k (a_k) or along m; one vector load when all
+    // 8 are in range and aligned
+#pragma unroll
+    for (int slab = 0; slab < GL_BK / 16; ++slab) {
+      const int mm = a_k ? (tid >> 1) : (tid & 15) * 8, kk = 16 * slab + (a_k ? (tid & 1) * 8 : (tid >> 4));
       const int m = m0 + mm, k = k0 + kk;
       const AT* src = A + (int64_t)m * t.sam + (int64_t)k * t.sak;
       const bool full = a_k ? (m < t.M && k + 7 < ke) : (m + 7 < t.M && k < ke);
@@ -83,8 +85,9 @@ __device__ __forceinline__ void gl_tile(const AT* __restrict__ A, const BT* __re
         else As[kk][mm + e] = v[e];
       }
     }
-    {
-      const int nn = b_n ? (tid & 15) * 8 : (tid >> 1), kk = b_n ? (tid >> 4) : (tid & 1) * 8;
+#pragma unroll
+    for (int slab = 0; slab < GL_BK / 16; ++slab) {
+      const int nn = b_n ? (tid & 15) * 8 : (tid >> 1), kk = 16 * slab + (b_n ? (tid >> 4) : (tid & 1) * 8);
       const int n = n0 + nn, k = k0 + kk;
       const BT* src = B + (int64_t)k * t.sbk + (int64_t)n * t.sbn;
       const bool full = b_n ? (n + 7 < t.N && k < ke) : (n < t.N && k + 7 < ke);

@@ -178,6 +178,62 @@ __device__ __forceinline__ void pass_doubling(bool fwd, VT* __restrict__ rho, co
   acc[0] = a1; acc[1] = b1; acc[2] = a2; acc[3] = b2; acc[4] = a3; acc[5] = b3;
 }
 
+// out = M x for the column-major D x D matrix M, summing j = 0..D-1 in order
+// like Eigen's row product.  Kept out of line: inside the state machine the
+// register pressure made the compiler wait for each column element before
+// loading the next; here the loads of four columns issue together.  Rows past
+// D read a valid element (row D-1) and are never stored, so the loads carry
+// no branches.
+template <int TPC, class VT>
+__device__ __noinline__ void dense_matvec(const double* __restrict__ M, const VT* __restrict__ x,
+                                          VT* __restrict__ out, int D, int tid) {
+  constexpr int RB = TPC >= 256 ? 4 : 8;  // TPC * RB rows per pass (dense_e: D <= 1024)
+  for (int i0 = 0; i0 < D; i0 += TPC * RB) {
+    double acc[RB];
+    int ii[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      acc[r] = 0.0;
+      const int i = i0 + tid + TPC * r;
+      ii[r] = i < D ? i : D - 1;
+    }
+    int j = 0;
+    for (; j + 4 <= D; j += 4) {
+      double v[4][RB], xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xv[u] = x[j + u];
+        const double* col = M + (int64_t)(j + u) * D;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) v[u][r] = col[ii[r]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r] += v[u][r] * xv[u];
+    }
+    for (; j < D; ++j) {
+      const double xj = x[j];
+      const double* col = M + (int64_t)j * D;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) acc[r] += col[ii[r]] * xj;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int i = i0 + tid + TPC * r;
+      if (i < D) out[i] = acc[r];
+    }
+  }
+}
+
+// col += w * d (the dense Welford covariance update, one column per call)
+template <int TPC, class VT>
+__device__ __forceinline__ void pass_axpy(double* __restrict__ col, const VT* __restrict__ w, double d, int D,
+                                          int tid) {
+#pragma unroll 4
+  for (int i = tid; i < D; i += TPC) col[i] = col[i] + (double)w[i] * d;
+}
+
 template <int METRIC, class GradFn, int TPC = NUTS_WAVE, class VT = double>
 struct NutsWave {
   static_assert(TPC % NUTS_WAVE == 0 && TPC <= 256, "threads per chain");
@@ -293,32 +349,10 @@ struct NutsWave {
     if (DENSE) return xs[i];
     return METRIC == METRIC_UNIT ? (double)x[i] : (double)minv[i] * (double)x[i];
   }
-  // out = Minv x (dense), summing j = 0..D-1 in order like Eigen's row product
+  // out = Minv x (dense)
   __device__ void matvec(const VT* x, VT* out) const {
     gsync();
-    constexpr int RB = 8;
-    for (int i0 = 0; i0 < D; i0 += TPC * RB) {
-      double acc[RB];
-#pragma unroll
-      for (int r = 0; r < RB; ++r) acc[r] = 0.0;
-      const VT* __restrict__ xr = x;
-      const double* __restrict__ Mr = dM;
-#pragma unroll 4
-      for (int j = 0; j < D; ++j) {
-        const double xj = xr[j];
-        const double* col = Mr + (int64_t)j * D;  // column j = row j of Minv
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-          const int i = i0 + tid + TPC * r;
-          if (i < D) acc[r] += col[i] * xj;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        const int i = i0 + tid + TPC * r;
-        if (i < D) out[i] = acc[r];
-      }
-    }
+    dense_matvec<TPC, VT>(dM, x, out, D, tid);
     gsync();
   }
   // T = 1/2 p·Minv p; for dense ps must hold Minv p
@@ -800,9 +834,8 @@ struct NutsWave {
         } converge();
         gsync();
         for (int j = 0; j < D; ++j) {
-          const double dj = delta[j];
-          double* col = dW + (int64_t)j * D;
-          for (int i = tid; i < D; i += TPC) col[i] = col[i] + (double)wm2[i] * dj; converge();
+          pass_axpy<TPC>(dW + (int64_t)j * D, wm2, (double)delta[j], D, tid);
+          converge();
         }
       }
     }

@@ -127,7 +127,8 @@ def test_logistic_lockstep_matches_oracle_posterior():
     gm = ca.Model("logistic_regression", data)
     fit = gm.sample(num_chains=64, seed=3, fused=0)
     assert fit.leaf_steps > 0
-    r = O.Model("logistic_regression", data).run(seed=3, num_chains=4)
+    # 16 oracle chains: its MCSE, not the GPU's, dominates the 4-MCSE band
+    r = O.Model("logistic_regression", data).run(seed=3, num_chains=16)
     for j in range(6):
         name = f"beta.{j + 1}"
         xg = fit.param(name)

@@ -21,7 +21,7 @@ namespace nutsgpu {
 __device__ __forceinline__ double hl_log1p_exp(double e, double* sig) {
   const double t = exp(-fabs(e));
   *sig = e >= 0.0 ? 1.0 / (1.0 + t) : t / (1.0 + t);
-  return (e > 0.0 ? e : 0.0) + log1p(t);
+  return (e > 0.0 ? e : 0.0) + log1p_unit(t);
 }
 
 __global__ void __launch_bounds__(256) hl_group_kernel(HlArgs a) {

@@ -112,7 +112,7 @@ def bench_c1(args):
     cpu_ess = E.summary(r["draws"][:, :, 7].T)["ess_bulk"]
     return {"metric": "min bulk-ESS/sec (bernoulli, 4 chains)", "value": ess / fit.sampling_seconds, "unit": "ESS/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True, "dtype": dtype,
+            "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True, "dtype": "f64",
             "data": "examples/bernoulli/bernoulli.data.json",
             "config": {"workload": "bernoulli N=10, 4 chains, diag_e, fused per-chain kernel"},
             "grad_evals_per_sec": float(arr[:, :, 4].sum()) / fit.sampling_seconds,

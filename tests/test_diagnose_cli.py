@@ -1,7 +1,8 @@
 """The diagnose CLI on the reference's example output: its report equals the
 golden text of diagnose_test.cpp (tree depth saturation, divergences, E-BFMI,
 ESS per draw and R-hat checks; CPU only).  Also: stansummary and diagnose read
-only the sampling draws of a file with saved warmup."""
+only the sampling draws of a file with saved warmup, and the CLI writes the
+save_cmdstan_config JSON."""
 import os
 import subprocess
 
@@ -55,3 +56,21 @@ def test_saved_warmup_rows_are_skipped(tmp_path):
                          os.path.join(GOLD, "bernoulli_chain_1.csv")], capture_output=True, text=True, timeout=60)
     assert sa.returncode == 0 and sb.returncode == 0
     assert (tmp_path / "a.csv").read_text() == (tmp_path / "b.csv").read_text()
+
+
+def test_config_json_is_written_before_sampling(tmp_path):
+    """save_cmdstan_config=1 writes <output base>_config.json from the argument
+    tree before the model is built (so also where no GPU is present): valid
+    JSON with typed values, lists as {"value": choice, choice: {...}}."""
+    import json
+    exe = os.path.join(BIN, "bernoulli")
+    subprocess.run([exe, "sample", "num_samples=33", "adapt", "delta=0.9", "algorithm=hmc", "engine=nuts",
+                    "max_depth=7", "output", "file=run.csv", "save_cmdstan_config=1", "random", "seed=5"],
+                   cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    cfg = json.loads((tmp_path / "run_config.json").read_text())
+    s = cfg["method"]["sample"]
+    assert cfg["method"]["value"] == "sample" and s["num_samples"] == 33 and s["num_warmup"] == 1000
+    assert s["adapt"]["delta"] == 0.9 and s["adapt"]["engaged"] is True
+    assert s["algorithm"]["hmc"]["engine"] == {"value": "nuts", "nuts": {"max_depth": 7}}
+    assert cfg["random"]["seed"] == 5 and cfg["output"]["file"] == "run.csv"
+    assert cfg["data"]["file"] == "" and cfg["mpi_enabled"] is False

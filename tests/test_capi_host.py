@@ -119,3 +119,25 @@ def test_transform_inits_maps_constrained_values():
         es.transform_inits({"tau": -1.0})
     with pytest.raises(ca.SamplerError, match="theta"):
         bern.transform_inits({"theta": 1.5})
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N > 1 exits with a message: the build is one-GPU (the
+    chain-sharded design is documented, not built)."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["n_gpus"] == 2 and "one GPU" in out["error"]
+
+
+def test_precision_fp32_config_validates():
+    """precision 1 is accepted by the config check (FP32 contraction for the
+    logistic regression, FP32 leapfrog on the per-chain kernel); 3 is not."""
+    assert ca.lib().nuts_config_validate(C.byref(_cfg(precision=1))) == 0
+    assert ca.lib().nuts_config_validate(C.byref(_cfg(precision=3))) == 1
+    assert "precision" in ca.last_error()

@@ -24,6 +24,7 @@
 #include <iostream>
 #include <map>
 #include <memory>
+#include <regex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -170,11 +171,8 @@ void print_tree(const Node* n, int depth, std::ostream& o, const std::string& pr
 // values are numbers, booleans or strings.
 std::string json_value(const std::string& v) {
   if (v == "true" || v == "false") return v;
-  char* e = nullptr;
-  if (!v.empty() && v.find_first_not_of("0123456789+-.eE") == std::string::npos) {
-    std::strtod(v.c_str(), &e);
-    if (*e == '\0') return v;
-  }
+  static const std::regex number("-?(0|[1-9][0-9]*)(\\.[0-9]+)?([eE][+-]?[0-9]+)?");
+  if (std::regex_match(v, number)) return v;  // a JSON number as written
   std::string o = "\"";
   for (char ch : v) {
     if (ch == '"' || ch == '\\') o += '\\';

@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only (tests/, smoke check, bench cpu_baseline).
 //
 // FP64 CPU restatement of Stan's NUTS sampling path, written literally (recursive
-// tree, Eigen-free loops) so that it can judge the batched HIP implementation.
+// tree, Eigen-free loops) so that it can check the batched HIP implementation.
 // The upstream code (stan-dev/stan 2.36, src/stan/mcmc/ and src/stan/services/)
 // is not in the reference tree (the `stan` submodule is empty, .gitmodules:1-3);
 // this follows the published algorithm (Hoffman & Gelman 2014; Betancourt 2017,

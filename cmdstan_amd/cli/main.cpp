@@ -336,7 +336,11 @@ int run(int argc, char** argv) {
   long long seed = to_int(val(R, {"random", "seed"}), "seed");
   if (seed < -1 || seed > 4294967295LL)
     throw std::invalid_argument("seed must be in [-1, 4294967295]");  // arg_seed.hpp:19-47
-  if (seed == -1) seed = (long long)(std::chrono::system_clock::now().time_since_epoch().count() % 4294967296LL);
+  if (seed == -1) {
+    // arg_seed: a time-based seed, echoed in the configuration and the CSV header
+    seed = (long long)(std::chrono::system_clock::now().time_since_epoch().count() % 4294967296LL);
+    tree->kid("random")->kid("seed")->value = std::to_string(seed);
+  }
   cfg.seed = (uint64_t)seed;
   const std::string init = val(R, {"init"});
   char* ie = nullptr;

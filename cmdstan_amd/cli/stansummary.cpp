@@ -55,6 +55,7 @@ int usage() {
   std::cout << "Usage: stansummary [OPTIONS] stan_csv_file(s)\n"
                "  -c, --csv_filename [file]   Write statistics to a csv file.\n"
                "  -p, --percentiles [values]  Percentiles to report, default \"5,50,95\".\n"
+               "  -a, --autocorr [n]          Display the chain autocorrelation for the n-th input file.\n"
                "  -s, --sig_figs [n]          Significant figures reported. Default is 2.\n"
                "  -i, --include_param [name]  Include the named parameter in the summary output.\n";
   return 0;
@@ -67,6 +68,7 @@ int main(int argc, char** argv) {
   std::string csv_filename, pct_spec = "5,50,95";
   bool have_csv = false;
   int sig_figs = 2;
+  int autocorr = 0;  // 1-based input file whose autocorrelations are printed
   try {
     for (int i = 1; i < argc; ++i) {
       std::string a = argv[i];
@@ -88,11 +90,21 @@ int main(int argc, char** argv) {
           std::cout << "--sig_figs: Value " << v << " not in range 1 to 18" << std::endl;
           return 1;
         }
+      } else if ((v = value("--autocorr", "-a")) != "\x01") {
+        autocorr = std::atoi(v.c_str());
+        if (autocorr <= 0) {
+          std::cout << "--autocorr: Number less or equal to 0" << std::endl;
+          return 1;
+        }
       } else if ((v = value("--include_param", "-i")) != "\x01") include.push_back(v);
       else if (!a.empty() && a[0] == '-') throw std::invalid_argument("unknown option " + a);
       else files.push_back(a);
     }
     if (files.empty()) return usage(), 1;
+    if (autocorr > (int)files.size()) {
+      std::cout << "Option --autocorr: " << autocorr << " not a valid chain id." << std::endl;
+      return 1;
+    }
     std::vector<std::string> pcts;
     for (auto& p : split(pct_spec, ',')) {
       const std::string t = trim(p);
@@ -180,6 +192,50 @@ int main(int argc, char** argv) {
               << ".\nFor each parameter, ESS_bulk and ESS_tail measure the effective sample size for the entire "
                  "sample (bulk) and for the .05 and .95 tails (tail), \nand R_hat measures the potential scale "
                  "reduction on split chains. At convergence R_hat will be very close to 1.00.\n";
+    if (autocorr > 0) {
+      // normalised autocorrelation of every reported column of one chain,
+      // lags 0 .. n-1 (stansummary_helper.hpp:700-745 layout)
+      const auto& ch = chains[autocorr - 1];
+      std::cout << "Displaying the autocorrelations for chain " << autocorr << ":" << std::endl << std::endl;
+      const int nd = (int)n;
+      int lag_w = 1;
+      for (int x = nd; x != 0; x /= 10) ++lag_w;
+      size_t wn = 0;
+      for (int p = 0; p < P; ++p) wn = std::max(wn, names[cols[p]].size());
+      std::vector<std::vector<double>> ac(P, std::vector<double>(nd, 0.0));
+      for (int p = 0; p < P; ++p) {
+        double m = 0.0, lo = ch.rows[0][cols[p]], hi = lo;
+        for (int k = 0; k < nd; ++k) {
+          const double x = ch.rows[k][cols[p]];
+          m += x;
+          lo = std::min(lo, x);
+          hi = std::max(hi, x);
+        }
+        m /= nd;
+        if (hi == lo) {  // a constant column has no autocorrelation (0 / 0)
+          for (int lag = 0; lag < nd; ++lag) ac[p][lag] = std::nan("");
+          continue;
+        }
+        std::vector<double> d(nd);
+        for (int k = 0; k < nd; ++k) d[k] = ch.rows[k][cols[p]] - m;
+        double c0 = 0.0;
+        for (int k = 0; k < nd; ++k) c0 += d[k] * d[k];
+        for (int lag = 0; lag < nd; ++lag) {
+          double c = 0.0;
+          for (int k = 0; k + lag < nd; ++k) c += d[k] * d[k + lag];
+          ac[p][lag] = c / c0;
+        }
+      }
+      std::cout << std::setprecision(6) << std::defaultfloat;
+      std::cout << std::setw(lag_w > 4 ? lag_w : 4) << "Lag";
+      for (int p = 0; p < P; ++p) std::cout << std::setw((int)wn + 1) << std::right << names[cols[p]];
+      std::cout << std::endl;
+      for (int lag = 0; lag < nd; ++lag) {
+        std::cout << std::setw(lag_w) << std::right << lag;
+        for (int p = 0; p < P; ++p) std::cout << std::setw((int)wn + 1) << std::right << ac[p][lag];
+        std::cout << std::endl;
+      }
+    }
     if (have_csv) {
       std::ofstream out(csv_filename);
       if (!out) {

@@ -41,3 +41,28 @@ def test_bad_sig_figs_is_an_error(tmp_path):
     r = subprocess.run([BIN, "--sig_figs", "-1", CSV], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "--sig_figs: Value -1 not in range" in r.stdout
+
+
+def test_autocorr_option(tmp_path):
+    """--autocorr n (stansummary_test.cpp:345-375 error cases; report layout of
+    stansummary_helper.hpp:700-745): lag 0 is 1, the lags follow numpy's
+    correlation of the centred draws, constant columns print nan."""
+    import numpy as np
+    from tests.conftest import load_stan_csv
+    for arg, msg in (("-1", "--autocorr: Number less or equal to 0"), ("0", "--autocorr: Number less or equal to 0"),
+                     ("2", "not a valid chain id")):
+        r = subprocess.run([BIN, "--autocorr", arg, CSV], capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and msg in r.stdout
+    r = subprocess.run([BIN, "--autocorr", "1", CSV], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    lines = r.stdout.split("Displaying the autocorrelations for chain 1:")[1].strip().splitlines()
+    head = lines[0].split()
+    assert head[0] == "Lag" and head[-1] == "theta"
+    rows = [l.split() for l in lines[1:]]
+    names, d, _ = load_stan_csv(CSV)
+    x = d[:, names.index("theta")] - d[:, names.index("theta")].mean()
+    ref = np.correlate(x, x, mode="full")[len(x) - 1:] / np.dot(x, x)
+    got = np.array([float(rw[-1]) for rw in rows])
+    assert len(got) == len(x) and got[0] == 1.0
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-6)
+    assert rows[1][head.index("stepsize__")] == "nan"

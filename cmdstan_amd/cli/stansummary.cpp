@@ -105,13 +105,31 @@ int main(int argc, char** argv) {
       std::cout << "Option --autocorr: " << autocorr << " not a valid chain id." << std::endl;
       return 1;
     }
+    // percentiles: numbers in [0, 100], strictly increasing (stansummary.cpp:98-110)
     std::vector<std::string> pcts;
+    double prev = -1.0;
     for (auto& p : split(pct_spec, ',')) {
-      const std::string t = trim(p);
-      if (t.empty() || t == "\"\"") continue;
-      const double x = std::atof(t.c_str());
-      if (!(x >= 0 && x <= 100)) throw std::invalid_argument("Option --percentiles " + pct_spec + ": values must be in (0,100)");
+      std::string t = trim(p);
+      t.erase(std::remove(t.begin(), t.end(), '"'), t.end());
+      if (t.empty()) continue;
+      char* end = nullptr;
+      const double x = std::strtod(t.c_str(), &end);
+      if (*end != '\0' || !(x >= 0 && x <= 100)) {
+        std::cout << "Option --percentiles " << pct_spec << ": values must be in range [0, 100]" << std::endl;
+        return 1;
+      }
+      if (!(x > prev)) {
+        std::cout << "Option --percentiles " << pct_spec << ": values must be in ascending order" << std::endl;
+        return 1;
+      }
+      prev = x;
       pcts.push_back(t);
+    }
+    for (auto& f : files) {
+      if (!std::ifstream(f).good()) {
+        std::cout << "File does not exist: " << f << std::endl;
+        return 1;
+      }
     }
     std::vector<ChainFile> chains;
     for (auto& f : files) chains.push_back(read_csv(f));
@@ -123,6 +141,21 @@ int main(int argc, char** argv) {
     }
     if (n < 2) throw std::runtime_error("need at least 2 draws per chain");
     // column selection: sampler columns, then requested (or all) model columns
+    std::vector<std::string> unknown;
+    for (auto& inc : include) {
+      bool found = false;
+      for (auto& nm : names) {
+        const std::string dn = display_name(nm);
+        found = found || nm == inc || dn == inc || dn.rfind(inc + "[", 0) == 0;
+      }
+      if (!found) unknown.push_back(inc);
+    }
+    if (!unknown.empty()) {
+      std::cout << "--include_param: Unrecognized parameter(s): ";
+      for (auto& u : unknown) std::cout << "'" << u << "' ";
+      std::cout << std::endl;
+      return 1;
+    }
     std::vector<int> cols;
     for (size_t j = 0; j < names.size(); ++j) {
       const bool sampler = names[j].size() > 2 && names[j].compare(names[j].size() - 2, 2, "__") == 0;

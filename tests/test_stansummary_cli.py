@@ -66,3 +66,23 @@ def test_autocorr_option(tmp_path):
     assert len(got) == len(x) and got[0] == 1.0
     assert np.allclose(got, ref, rtol=1e-5, atol=1e-6)
     assert rows[1][head.index("stepsize__")] == "nan"
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--percentiles", "-1"], "Option --percentiles"),           # stansummary_test.cpp:375-406
+    (["--percentiles", "101"], "Option --percentiles"),
+    (["--percentiles", "2,30,5"], "Option --percentiles"),
+    (["-i", "psi"], "--include_param: Unrecognized parameter(s): 'psi' "),  # :408-424
+])
+def test_bad_arguments_are_errors(args, msg):
+    r = subprocess.run([BIN] + args + [CSV], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and msg in r.stdout
+
+
+def test_missing_file_and_no_arguments():
+    r = subprocess.run([BIN, "no_such.csv"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "File does not exist" in r.stdout   # :290-301
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=60)
+    assert "Usage: stansummary" in r.stdout                            # :279-288
+    r = subprocess.run([BIN, "--percentiles", "2,50,95", CSV], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "2%" in r.stdout and "95%" in r.stdout

@@ -73,6 +73,11 @@ Mat split(const double* x, int n, int m) {
 double ess(const Mat& x) {
   const int n = x.n, m = x.m;
   if (n < 4) return std::numeric_limits<double>::quiet_NaN();
+  // constant draws have no effective sample size (NaN, which the tail ESS's
+  // min() then passes over when it is the second operand)
+  bool varies = false;
+  for (double v : x.v) varies = varies || v != x.v[0];
+  if (!varies) return std::numeric_limits<double>::quiet_NaN();
   std::vector<double> mean(m), var(m);
   for (int j = 0; j < m; ++j) {
     double s = 0;

@@ -34,21 +34,47 @@ double quantile7(std::vector<double> v, double p) {
   return v[lo] + (h - lo) * (v[hi] - v[lo]);
 }
 
-std::string fmt_sig(double v, int sig) {
-  std::ostringstream os;
-  if (!std::isfinite(v)) {
-    os << v;
-    return os.str();
-  }
+// Fixed-point slot of one value at `sig` significant digits: its width and
+// decimal places (the column rule of stansummary_helper.hpp:20-48).
+void fixed_slot(double v, int sig, int& width, int& prec) {
   const double a = std::fabs(v);
-  if (a != 0 && (a >= 1e6 || a < 1e-3)) {
-    os << std::scientific << std::setprecision(std::max(0, sig - 1)) << v;
+  if (v == 0) {
+    width = sig;
+    prec = sig;
+  } else if (std::isnan(v)) {
+    width = 3;
+    prec = sig;
+  } else if (a >= 1) {
+    const int ip = (int)std::ceil(std::log10(a) + 1e-6);
+    width = ip >= sig ? ip : sig + 1;
+    prec = ip >= sig ? 0 : sig - ip;
   } else {
-    const int ip = a >= 1 ? (int)std::floor(std::log10(a)) + 1 : 0;
-    const int prec = std::max(0, sig - ip);
-    os << std::fixed << std::setprecision(ip >= sig ? 0 : prec) << v;
+    const int fp = (int)std::fabs(std::floor(std::log10(a)));
+    width = 1 + fp + sig;
+    prec = fp + sig - 1;
   }
-  return os.str();
+  if (std::signbit(v)) ++width;
+}
+
+// A column prints fixed-point when its widest value plus two spaces of padding
+// stays under 8 characters, else in scientific notation
+// (stansummary_helper.hpp:94-124); returns the column width.
+int column_format(const std::vector<double>& x, const std::string& name, int sig, bool& scientific) {
+  int wmax = 0;
+  bool neg = false;
+  for (double v : x) {
+    int w, pr;
+    fixed_slot(v, sig, w, pr);
+    wmax = std::max(wmax, w);
+    neg = neg || v < 0;
+  }
+  if (wmax + 2 < 8) {
+    scientific = false;
+    return std::max<int>(wmax, (int)name.size()) + 2;
+  }
+  scientific = true;
+  const int ws = sig + 1 + 4 + (neg ? 1 : 0);
+  return std::max<int>(ws, (int)name.size()) + 2;
 }
 
 int usage() {
@@ -200,27 +226,53 @@ int main(int argc, char** argv) {
     for (auto& c : chains) { tw += c.warmup_s; ts += c.sampling_s; }
     std::cout << std::fixed << std::setprecision(3) << "Warmup took " << tw << " seconds\nSampling took " << ts
               << " seconds\n\n" << std::defaultfloat;
-    size_t wname = 4;
+    // formats per column are chosen separately for the sampler rows and the
+    // model rows (lp__ prints with the model rows' formats); a column is as
+    // wide as the wider of the two (stansummary.cpp:205-246)
+    size_t wname = 0;
     for (int p = 0; p < P; ++p) wname = std::max(wname, display_name(names[cols[p]]).size());
-    std::vector<std::vector<std::string>> cells(P, std::vector<std::string>(header.size()));
-    std::vector<size_t> w(header.size());
-    for (size_t h = 0; h < header.size(); ++h) w[h] = header[h].size();
-    for (int p = 0; p < P; ++p)
-      for (size_t h = 0; h < header.size(); ++h) {
-        const bool ess = header[h] == "ESS_bulk" || header[h] == "ESS_tail";
-        cells[p][h] = ess ? (std::isfinite(table[p][h]) ? std::to_string((long long)std::llround(table[p][h])) : "nan")
-                          : fmt_sig(table[p][h], sig_figs);
-        w[h] = std::max(w[h], cells[p][h].size());
+    auto is_sampler = [&](int p) {
+      const std::string& nm = names[cols[p]];
+      return nm != "lp__" && nm.size() > 2 && nm.compare(nm.size() - 2, 2, "__") == 0;
+    };
+    auto is_model = [&](int p) { return names[cols[p]].find("__") == std::string::npos; };
+    std::vector<int> cw(header.size());
+    std::vector<char> sci_s(header.size()), sci_m(header.size());
+    for (size_t h = 0; h < header.size(); ++h) {
+      std::vector<double> cs, cm;
+      for (int p = 0; p < P; ++p) {
+        if (is_sampler(p)) cs.push_back(table[p][h]);
+        else if (is_model(p)) cm.push_back(table[p][h]);
       }
-    std::cout << std::string(wname, ' ');
-    for (size_t h = 0; h < header.size(); ++h) std::cout << "  " << std::setw((int)w[h]) << header[h];
-    std::cout << "\n\n";
-    for (int p = 0; p < P; ++p) {
-      std::cout << std::left << std::setw((int)wname) << display_name(names[cols[p]]) << std::right;
-      for (size_t h = 0; h < header.size(); ++h) std::cout << "  " << std::setw((int)w[h]) << cells[p][h];
-      std::cout << "\n";
-      if (names[cols[p]] == "energy__") std::cout << "\n";
+      bool a = false, b = false;
+      const int ws = column_format(cs, header[h], sig_figs, a);
+      const int wm = column_format(cm, header[h], sig_figs, b);
+      cw[h] = std::max(ws, wm);
+      sci_s[h] = a;
+      sci_m[h] = b;
     }
+    auto row = [&](int p, const std::vector<char>& sci) {
+      std::cout << std::setw((int)wname + 1) << std::left << display_name(names[cols[p]]) << std::right;
+      for (size_t h = 0; h < header.size(); ++h) {
+        const double v = table[p][h];
+        int w, pr;
+        fixed_slot(v, sig_figs, w, pr);
+        if (sci[h]) std::cout << std::scientific << std::setprecision(sig_figs - 1);
+        else std::cout << std::fixed << std::setprecision(pr);
+        std::cout << std::setw(cw[h]) << v;
+      }
+      std::cout << std::defaultfloat << "\n";
+    };
+    std::cout << std::setw((int)wname + 1) << "";
+    for (size_t h = 0; h < header.size(); ++h) std::cout << std::setw(cw[h]) << header[h];
+    std::cout << "\n\n";
+    for (int p = 0; p < P; ++p)
+      if (names[cols[p]] == "lp__") row(p, sci_m);
+    for (int p = 0; p < P; ++p)
+      if (is_sampler(p)) row(p, sci_s);
+    std::cout << "\n";
+    for (int p = 0; p < P; ++p)
+      if (is_model(p)) row(p, sci_m);
     std::cout << "\nSamples were drawn using " << meta("algorithm", "hmc") << " with " << meta("engine", "nuts")
               << ".\nFor each parameter, ESS_bulk and ESS_tail measure the effective sample size for the entire "
                  "sample (bulk) and for the .05 and .95 tails (tail), \nand R_hat measures the potential scale "

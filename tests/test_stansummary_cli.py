@@ -86,3 +86,17 @@ def test_missing_file_and_no_arguments():
     assert "Usage: stansummary" in r.stdout                            # :279-288
     r = subprocess.run([BIN, "--percentiles", "2,50,95", CSV], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "2%" in r.stdout and "95%" in r.stdout
+
+
+def test_console_rows_match_golden():
+    """The console table of stansummary_test.cpp:426-480 (check_console_output):
+    column formats chosen separately for the sampler rows and the model rows,
+    lp__ printed with the model rows' formats, widths from the wider block."""
+    r = subprocess.run([BIN, CSV], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    lines = r.stdout.split("\n")
+    for gold in ("lp__            -7.3  3.7e-02     0.77   0.30   -9.0  -7.0  -6.8       519       503    1.0",
+                 "accept_stat__   0.90  4.6e-03  1.5e-01  0.064   0.57  0.96   1.0      1284       941   1.00",
+                 "energy__         7.8  5.1e-02  1.0e+00   0.75    6.8   7.5   9.9       490       486    1.0",
+                 "theta           0.26  6.1e-03     0.12   0.12  0.080  0.25  0.47       362       396    1.0"):
+        assert gold in lines, gold

@@ -94,6 +94,7 @@ int main(int argc, char** argv) {
   std::string csv_filename, pct_spec = "5,50,95";
   bool have_csv = false;
   int sig_figs = 2;
+  bool sig_figs_given = false;  // the csv file keeps 6 digits unless --sig_figs is given
   int autocorr = 0;  // 1-based input file whose autocorrelations are printed
   try {
     for (int i = 1; i < argc; ++i) {
@@ -112,6 +113,7 @@ int main(int argc, char** argv) {
       else if ((v = value("--percentiles", "-p")) != "\x01") pct_spec = v;
       else if ((v = value("--sig_figs", "-s")) != "\x01") {
         sig_figs = std::atoi(v.c_str());
+        sig_figs_given = true;
         if (sig_figs < 1 || sig_figs > 18) {
           std::cout << "--sig_figs: Value " << v << " not in range 1 to 18" << std::endl;
           return 1;
@@ -218,14 +220,53 @@ int main(int argc, char** argv) {
     // console
     const ChainFile& c0 = chains[0];
     auto meta = [&](const std::string& k, const std::string& d) { return c0.meta.count(k) ? c0.meta.at(k) : d; };
-    std::cout << "Inference for Stan model: " << meta("model", "model") << "\n"
-              << M << " chains: each with iter=" << meta("num_samples", std::to_string(n))
-              << "; warmup=" << meta("num_warmup", "0") << "; thin=" << meta("thin", "1") << "; " << n * M
-              << " iterations saved.\n\n";
-    double tw = 0, ts = 0;
-    for (auto& c : chains) { tw += c.warmup_s; ts += c.sampling_s; }
-    std::cout << std::fixed << std::setprecision(3) << "Warmup took " << tw << " seconds\nSampling took " << ts
-              << " seconds\n\n" << std::defaultfloat;
+    // model, chain counts and timing (stansummary_helper.hpp:595-680), then the
+    // closing notes (:682-697); the csv file repeats them as "# " comments
+    auto write_timing = [&](const std::string& pre, std::ostream& o) {
+      o << pre << "Inference for Stan model: " << meta("model", "model") << "\n"
+        << pre << M << " chains: each with iter=" << meta("num_samples", std::to_string(n)) << "; warmup="
+        << meta("num_warmup", "0") << "; thin=" << meta("thin", "1") << "; " << n * M << " iterations saved.\n"
+        << pre << "\n";
+      auto took = [&](const char* what, bool warm) {
+        double tot = 0;
+        for (auto& c : chains) tot += warm ? c.warmup_s : c.sampling_s;
+        std::string unit = "seconds";
+        if (tot / 3600 > 1) { tot /= 3600; unit = "hours"; }
+        else if (tot / 60 > 1) { tot /= 60; unit = "minutes"; }
+        auto fx = [&](double v) {
+          int w, pr;
+          fixed_slot(v, 2, w, pr);
+          o << std::fixed << std::setprecision(pr) << v;
+        };
+        o << pre << what << " took ";
+        if (chains.size() == 1) {
+          fx(tot);
+          o << " " << unit << "\n";
+        } else {
+          o << "(";
+          for (size_t c = 0; c < chains.size(); ++c) {
+            if (c) o << ", ";
+            fx(warm ? chains[c].warmup_s : chains[c].sampling_s);
+          }
+          o << ") seconds, ";
+          fx(tot);
+          o << " " << unit << " total\n";
+        }
+        o << std::defaultfloat;
+      };
+      took("Warmup", true);
+      took("Sampling", false);
+    };
+    auto write_sampler_info = [&](const std::string& pre, std::ostream& o) {
+      o << pre << "Samples were drawn using " << meta("algorithm", "hmc") << " with " << meta("engine", "nuts")
+        << ".\n"
+        << pre << "For each parameter, ESS_bulk and ESS_tail measure the effective sample size for the entire "
+                  "sample (bulk) and for the .05 and .95 tails (tail), \n"
+        << pre << "and R_hat measures the potential scale reduction on split chains. At convergence R_hat will "
+                  "be very close to 1.00.\n";
+    };
+    write_timing("", std::cout);
+    std::cout << "\n";
     // formats per column are chosen separately for the sampler rows and the
     // model rows (lp__ prints with the model rows' formats); a column is as
     // wide as the wider of the two (stansummary.cpp:205-246)
@@ -273,10 +314,8 @@ int main(int argc, char** argv) {
     std::cout << "\n";
     for (int p = 0; p < P; ++p)
       if (is_model(p)) row(p, sci_m);
-    std::cout << "\nSamples were drawn using " << meta("algorithm", "hmc") << " with " << meta("engine", "nuts")
-              << ".\nFor each parameter, ESS_bulk and ESS_tail measure the effective sample size for the entire "
-                 "sample (bulk) and for the .05 and .95 tails (tail), \nand R_hat measures the potential scale "
-                 "reduction on split chains. At convergence R_hat will be very close to 1.00.\n";
+    std::cout << "\n";
+    write_sampler_info("", std::cout);
     if (autocorr > 0) {
       // normalised autocorrelation of every reported column of one chain,
       // lags 0 .. n-1 (stansummary_helper.hpp:700-745 layout)
@@ -330,12 +369,14 @@ int main(int argc, char** argv) {
       out << "name";
       for (auto& h : header) out << "," << h;
       out << "\n";
-      out << std::setprecision(6);
+      out << std::setprecision(sig_figs_given ? sig_figs : 6);
       for (int p = 0; p < P; ++p) {
         out << "\"" << display_name(names[cols[p]]) << "\"";
         for (double v : table[p]) out << "," << v;
         out << "\n";
       }
+      write_timing("# ", out);
+      write_sampler_info("# ", out);
     }
   } catch (const std::exception& e) {
     std::cout << e.what() << std::endl;

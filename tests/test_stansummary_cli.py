@@ -100,3 +100,36 @@ def test_console_rows_match_golden():
                  "energy__         7.8  5.1e-02  1.0e+00   0.75    6.8   7.5   9.9       490       486    1.0",
                  "theta           0.26  6.1e-03     0.12   0.12  0.080  0.25  0.47       362       396    1.0"):
         assert gold in lines, gold
+
+
+def test_csv_output_sig_figs_golden(tmp_path):
+    """stansummary_test.cpp:572-620: with --sig_figs 2 the csv rows use 2
+    significant digits."""
+    out = tmp_path / "s.csv"
+    r = subprocess.run([BIN, f"--csv_filename={out}", "--sig_figs", "2", CSV], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0
+    lines = out.read_text().splitlines()
+    assert lines[1] == '"lp__",-7.3,0.037,0.77,0.3,-9,-7,-6.8,5.2e+02,5e+02,1'
+    assert lines[7] == '"energy__",7.8,0.051,1,0.75,6.8,7.5,9.9,4.9e+02,4.9e+02,1'
+    assert lines[8] == '"theta",0.26,0.0061,0.12,0.12,0.08,0.25,0.47,3.6e+02,4e+02,1'
+
+
+def test_csv_output_include_param_golden(tmp_path):
+    """stansummary_test.cpp:622-700 on the eight-schools fixture: selected
+    parameters by either naming, then the "# " summary comments."""
+    out = tmp_path / "e.csv"
+    es = os.path.join(ROOT, "tests", "golden", "eight_schools_output.csv")
+    r = subprocess.run([BIN, f"--csv_filename={out}", "--include_param", "theta.6", "-i", "theta[7]", es],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    lines = out.read_text().splitlines()
+    assert lines[1] == ('"lp__",-15.5617,0.97319,6.05585,6.3817,-25.3182,-15.7598,-5.47732,41.1897,113.537,'
+                        '1.00153')
+    assert lines[7] == ('"energy__",20.5888,1.01449,6.43127,6.6161,10.2809,20.8278,30.9921,42.5605,140.171,'
+                        '1.00069')
+    assert lines[8] == ('"theta[6]",5.001,0.365016,5.76072,5.37947,-4.95375,5.22746,14.1688,230.645,464.978,'
+                        '1.00054')
+    assert lines[9] == ('"theta[7]",8.54125,0.650098,6.22195,5.35785,-0.814388,8.09342,19.2622,92.3075,241.177,'
+                        '1.00244')
+    assert lines[10] == "# Inference for Stan model: eight_schools_cp_model"

@@ -110,6 +110,29 @@ int oracle_transition_replay(void* m, int metric, const double* minv, double nom
   }
 }
 
+// base_hmc::init_stepsize from q0 with injected normals (one momentum draw per
+// probe).  out = {final nominal step size, normals used}.
+int oracle_init_stepsize(void* m, int metric, const double* minv, double nom_eps, const double* q0, const double* z,
+                         int64_t nz, double* out) {
+  try {
+    Model* mm = static_cast<Model*>(m);
+    const int D = mm->dim();
+    InjectedDraws draws(nullptr, 0, z, (size_t)nz);
+    NutsSampler s(mm, metric, &draws);
+    if (metric != UNIT_E) for (size_t i = 0; i < s.ham.minv.size(); ++i) s.ham.minv[i] = minv[i];
+    s.nom_eps = nom_eps;
+    s.z.q.assign(q0, q0 + D);
+    s.ham.update_potential_gradient(s.z);
+    s.init_stepsize();
+    out[0] = s.nom_eps;
+    out[1] = (double)draws.iz;
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
 // Dual averaging on a given accept-stat sequence (deterministic recurrence).
 void oracle_dual_averaging(double mu, double delta, double gamma, double kappa, double t0, const double* accept,
                            int n, double* eps_out, double* x_bar_out) {

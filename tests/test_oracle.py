@@ -1,5 +1,7 @@
 """The FP64 CPU oracle, pinned against the reference's fixtures, known-answer
 vectors and closed forms (no GPU needed)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -197,6 +199,137 @@ def test_transition_invariants_and_divergence():
     # a huge step size diverges on the first leaf
     q, info = m.transition_replay(np.ones(4), [0.9] * 10, np.ones(4), 50.0, metric=1)
     assert info["divergent"] == 1 and info["n_leapfrog"] == 1 and info["depth"] == 0
+
+
+def _gauss_momenta(q0, p0, w, eps, n):
+    """Momenta p_0..p_n of n forward leapfrog steps on V = sum_i w_i q_i^2 / 2
+    with the unit metric.  Per coordinate one step is the linear map
+    (q, p) -> [[1 - e^2 w / 2, e], [-e w (1 - e^2 w / 4), 1 - e^2 w / 2]] (q, p)."""
+    q, p = np.array(q0, float), np.array(p0, float)
+    w = np.asarray(w, float)
+    a = 1 - eps * eps * w / 2
+    ps = [p.copy()]
+    for _ in range(n):
+        q, p = a * q + eps * p, -eps * w * (1 - eps * eps * w / 4) * q + a * p
+        ps.append(p.copy())
+    return np.array(ps)
+
+
+# Two trajectories on a D=2 Gaussian (precisions w, unit metric, every
+# doubling forward) where Stan's generalized U-turn checks across the two
+# halves of a merge stop the tree while the check over the whole merged span
+# alone would let it grow.  In each, the merged span [a, b) with halves
+# [a, m) and [m, b) has p_a . rho > 0 and p_{b-1} . rho > 0 for rho = sum of
+# its momenta, but p_{m} . (rho_[a,m) + p_m) or p_{m-1} . (rho_[m,b) + p_{m-1})
+# is negative.
+UTURN_TOP = dict(w=[1.0, 9.0], eps=0.5, q0=[1.4, -1.1], p0=[1.3, -0.6], span=(0, 2, 4), n_lf=3, depth=2,
+                 n_lf_without=7)   # top-level merge of the second doubling: old {0, 1} + new {2, 3}
+UTURN_INNER = dict(w=[1.0, 25.0], eps=0.3, q0=[-0.6, 0.9], p0=[-1.2, 0.3], span=(4, 6, 8), n_lf=7, depth=2,
+                   n_lf_without=15)  # root merge of the third doubling's subtree {4, 5} + {6, 7}
+
+
+def _uturn_dots(case):
+    a, m, b = case["span"]
+    ps = _gauss_momenta(case["q0"], case["p0"], case["w"], case["eps"], b)
+    rho = ps[a:b].sum(0)
+    whole = (ps[a] @ rho, ps[b - 1] @ rho)
+    left = ps[a:m].sum(0) + ps[m]
+    right = ps[m:b].sum(0) + ps[m - 1]
+    cross = (ps[a] @ left, ps[m] @ left, ps[m - 1] @ right, ps[b - 1] @ right)
+    return whole, cross
+
+
+@pytest.mark.parametrize("case", [UTURN_TOP, UTURN_INNER], ids=["top_level", "inside_subtree"])
+def test_uturn_cross_checks_decide(case):
+    whole, cross = _uturn_dots(case)
+    assert whole[0] > 0 and whole[1] > 0          # the check over the merged span passes
+    assert min(cross) < 0                          # a check across its halves fails
+    m = O.Model("gauss", dict(D=2, P=np.diag(case["w"])))
+    _, info = m.transition_replay(case["q0"], [0.9] * 64, case["p0"], case["eps"], metric=0, max_depth=6)
+    assert info["n_leapfrog"] == case["n_lf"] and info["depth"] == case["depth"]
+    assert info["divergent"] == 0
+
+
+def _oracle_variant(tmp_path, edit):
+    """The oracle rebuilt from a copy of its sources with `edit` applied to
+    nuts.hpp (a planted mistake, to show a pin test catches it)."""
+    import shutil
+    import subprocess
+    src = os.path.join(os.path.dirname(O.__file__))
+    for f in ("oracle_capi.cpp", "nuts.hpp", "models.hpp", "philox.hpp"):
+        shutil.copy(os.path.join(src, f), tmp_path / f)
+    text = (tmp_path / "nuts.hpp").read_text()
+    new = edit(text)
+    assert new != text
+    (tmp_path / "nuts.hpp").write_text(new)
+    so = tmp_path / "liboracle_variant.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-shared", "-o", str(so),
+                    str(tmp_path / "oracle_capi.cpp")], check=True)
+    return O.load(str(so))
+
+
+def test_uturn_pins_catch_dropped_cross_checks(tmp_path):
+    """With the checks across subtree halves deleted from the oracle, both
+    trees above grow further: the pins fail on that mistake."""
+    import re
+    L = _oracle_variant(tmp_path, lambda t: re.sub(r"persist &= criterion\([^;]*\);", ";", t))
+    for case in (UTURN_TOP, UTURN_INNER):
+        m = O.Model("gauss", dict(D=2, P=np.diag(case["w"])), library=L)
+        _, info = m.transition_replay(case["q0"], [0.9] * 64, case["p0"], case["eps"], metric=0, max_depth=6)
+        assert info["n_leapfrog"] == case["n_lf_without"] != case["n_lf"]
+
+
+# base_hmc::init_stepsize on the standard normal from q0 = 0 with the unit
+# metric: one leapfrog of size e from (0, p) ends at (e p, p (1 - e^2 / 2)), so
+# the probe's H0 - h = -p^2 e^4 / 8, and the step is kept when that exceeds
+# log 0.8, i.e. p^2 e^4 < 1.785.  Every probe draws a fresh momentum.
+@pytest.mark.parametrize("eps0,normals,expect_eps,expect_used", [
+    (1.0, [1.0] * 8, 2.0, 3),       # 1/8 ok -> up; at 1: ok -> 2; at 2: 16/8 too big -> stop at 2
+    (1.0, [0.5] * 8, 2.0, 3),       # p^2 = 1/4: e^4 < 7.14 -> 1 ok, 2 (e^4 = 16) too big
+    (4.0, [1.0] * 8, 1.0, 4),       # 4: too big -> down; 4 -> 2 -> 1, at 1 the probe is acceptable
+    (1.0, [0.1, 3.0, 0.1, 0.1], 1.0, 2),  # direction up from p = 0.1; the second probe (p = 3) is too big
+    (0.25, [2.0] * 8, 1.0, 4),      # p^2 e^4: 4/256 ok -> up; at 0.25 ok -> 0.5; 1/4 at 0.5 ok -> 1; 4 at 1 too big
+])
+def test_init_stepsize_by_hand(eps0, normals, expect_eps, expect_used):
+    m = O.Model("gauss", dict(D=1))
+    eps, used = m.init_stepsize([0.0], normals, eps0, metric=0)
+    assert eps == expect_eps and used == expect_used
+
+
+def test_init_stepsize_improper_limit():
+    # zero momenta: every probe keeps H, the step doubles past 1e7
+    m = O.Model("gauss", dict(D=1))
+    with pytest.raises(RuntimeError, match="improper"):
+        m.init_stepsize([0.0], [0.0] * 64, 1.0, metric=0)
+
+
+def _two_sample_chi2(a, b, min_count=10):
+    from scipy.stats import chi2_contingency
+    vals = sorted(set(a.tolist()) | set(b.tolist()))
+    ta = np.array([np.sum(a == v) for v in vals])
+    tb = np.array([np.sum(b == v) for v in vals])
+    keep = (ta + tb) >= min_count
+    return chi2_contingency(np.vstack([ta[keep], tb[keep]]))[1]
+
+
+def test_tree_sizes_match_fixtures_at_their_adapted_step(golden):
+    """Adaptation off at each fixture's printed step size and inverse metric:
+    the oracle's tree depths (and, for bernoulli, leapfrog counts) have the
+    fixture rows' distribution (two-sample chi-square)."""
+    names, d, _ = load_stan_csv(f"{golden}/bernoulli_chain_1.csv")   # :40-43 adaptation block
+    r = O.Model("bernoulli", BERN).run(seed=5, num_chains=4, adapt=0, num_warmup=100, num_samples=2500,
+                                       stepsize=1.02703, inv_metric=[0.526879])
+    dr = r["draws"].reshape(-1, r["draws"].shape[2])
+    assert _two_sample_chi2(d[:, 3], dr[:, 3]) > 1e-3
+    assert _two_sample_chi2(d[:, 4], dr[:, 4]) > 1e-3
+    names, d8, _ = load_stan_csv(f"{golden}/eight_schools_output.csv")   # :40-41
+    mi = [13.2101, 0.318126, 36.4485, 29.83, 33.6813, 25.6287, 21.8178, 26.9628, 25.8657, 32.7886]
+    eight = dict(J=8, y=[28.0, 8, -3, 7, -1, 1, 18, 12], sigma=[15.0, 10, 16, 11, 9, 11, 10, 18], mu_scale=10.0,
+                 tau_scale=10.0)
+    r = O.Model("eight_schools_cp", eight).run(seed=5, num_chains=4, adapt=0, num_warmup=200, num_samples=2500,
+                                                stepsize=0.117114, inv_metric=mi)
+    dr = r["draws"].reshape(-1, r["draws"].shape[2])
+    assert _two_sample_chi2(d8[:, 3], dr[:, 3]) > 1e-3
 
 
 def test_fixture_tree_invariant(golden):

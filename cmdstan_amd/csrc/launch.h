@@ -59,8 +59,6 @@ struct GlmArgs {
   hipEvent_t ev_begin = nullptr;  // optional events recorded around the contraction kernel alone
   hipEvent_t ev_end = nullptr;
 };
-size_t glm_workspace_doubles(int N, int C, int D, int rows_per_chunk);
-int glm_grad_f64(const GlmArgs& a, hipStream_t st);
 
 // FP64 matrix-core version (the sampler's default for logistic regression).
 struct GlmPlan {

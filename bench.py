@@ -1,14 +1,27 @@
-"""Benchmark: plain FP64 NUTS for Bayesian logistic regression on one MI355X.
+"""Benchmark: NUTS for Bayesian logistic regression on one MI355X.
 
 Workload (the configuration the headline metric is quoted on): X 100000 x 100
 FP64 iid N(0,1); beta* ~ N(0, 1/100); y ~ Bernoulli(logistic(X beta*)); prior
-beta ~ N(0, 2.5^2); 1024 chains, diag_e, max_depth 10.
+beta ~ N(0, 2.5^2); 1024 chains, diag_e, max_depth 10, 1000 warmup + 1000
+draws.
 
-A step is one NUTS iteration of all chains.  --warmup W adaptation iterations
-run untimed; exactly --steps K sampling iterations are timed (GPU phase wall
-time, synchronised on both sides).  value = min bulk-ESS over the 100
-coefficients and lp__, divided by the sampling seconds.  Gradient evaluations
-per second count every leapfrog of every chain in the sampling phase.
+Steps.  A step is a block of S = max(50, ceil(1000 / K)) sampling iterations
+of all chains, so the K timed steps always hold at least the workload's 1000
+draws per chain (--steps 20: S = 50, 1000 draws).  The adaptation phase runs
+max(1000, W * S) iterations (the workload's 1000 warmup iterations, or more
+when --warmup W asks for more) and is not timed.  The timed region is the
+sampling phase: the library synchronises the GPU stream on both sides of it.
+
+value = min bulk-ESS over the 100 coefficients and lp__ divided by the
+sampling seconds, for plain FP64 NUTS.  It is null (with the reason) when the
+run has not converged: max R-hat >= 1.01.  Gradient evaluations per second
+count every leapfrog of every chain in the sampling phase.
+
+Second leg of the metric ("batched+learned vs plain"): the same workload with
+the FP32 contraction and the learned flow in automatic mode (trained on the
+warmup draws, kept only if its pilot raises min bulk-ESS per gradient and per
+second, else plain NUTS continues); its min bulk-ESS/s over sampling and over
+the whole job, and its ratio to the plain FP64 leg.
 
 cpu_baseline: the FP64 CPU oracle (a restatement of Stan's algorithm; CmdStan
 itself cannot be built offline) runs a bounded sample of post-warmup
@@ -66,11 +79,38 @@ def make_data(N, D, seed):
     return dict(N=N, D=D, X=X, y=y, prior_scale=2.5), beta
 
 
+def host_threads():
+    """Host threads this process may use: the CPU affinity set, capped by
+    OMP_NUM_THREADS when the host sets it (a shared GPU box gives each GPU a
+    CPU share and says so there)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def step_iterations(steps):
+    """Sampling iterations per step: K steps hold at least 1000 draws."""
+    return max(50, -(-1000 // max(1, steps)))
+
+
 def cpu_baseline(data, fit, ess_per_grad, seconds_budget=20.0):
     """Oracle transitions on host threads from GPU-adapted states."""
     from oracle import oracle as O
     om = O.Model("logistic_regression", data)
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     D = data["D"]
     n_trans = 4
     C = len(fit.draws)
@@ -87,10 +127,83 @@ def cpu_baseline(data, fit, ess_per_grad, seconds_budget=20.0):
         secs += r["seconds"].max()
     gps = grads / secs
     return dict(value=ess_per_grad * gps, unit="ESS/s (extrapolated)", cores=threads, kind="port",
-                grad_evals_per_sec=gps,
+                grad_evals_per_sec=gps, cpu_model=cpu_model(), host_nproc=os.cpu_count(),
                 sample=f"{threads} chains x {n_trans} post-warmup transitions per round of the FP64 oracle with the "
-                       f"GPU-adapted step size and metric, {secs:.1f} s CPU wall; ESS/s = GPU ESS per gradient x CPU "
+                       f"GPU-adapted step size and metric, one chain per thread on {threads} host threads (the "
+                       f"process's CPU share), {secs:.1f} s CPU wall; ESS/s = GPU ESS per gradient x CPU "
                        f"gradients/s")
+
+
+def leg_stats(fit, D):
+    """min bulk-ESS over the D coefficients and lp__, max R-hat, sampling
+    gradient evaluations, and the per-transition energy error when reported."""
+    from cmdstan_amd.summary import summarize
+    arr = np.stack(fit.draws)  # chains x draws x cols
+    params = np.concatenate([arr[:, :, 7:7 + D], arr[:, :, :1]], axis=2).transpose(2, 0, 1)
+    st = summarize(params)
+    out = dict(min_bulk_ess=float(np.nanmin(st[:, 7])), rhat_max=float(np.nanmax(st[:, 9])),
+               grad_evals=float(arr[:, :, 4].sum()))
+    ee = getattr(fit, "energy_error", None)
+    if ee is not None:
+        e = np.concatenate([np.asarray(x).ravel() for x in ee])
+        out["energy_error"] = dict(max_abs_dH_per_transition_mean=float(np.mean(e)),
+                                   max_abs_dH_per_transition_p99=float(np.quantile(e, 0.99)),
+                                   max_abs_dH_per_transition_max=float(np.max(e)))
+    return out
+
+
+def learned_leg(model, args, n_draws, plain, plain_samp_s):
+    """Batched FP32 contraction + learned flow (automatic: plain NUTS is the
+    fallback), same chains, seed and draw count as the plain FP64 leg."""
+    D = args.D
+    print("c2 batched fp32 + flow auto", file=sys.stderr, flush=True)
+    t0 = time.time()
+    fit = model.sample(num_chains=args.chains, num_warmup=1000, num_samples=n_draws, seed=args.seed,
+                       metric="diag_e", fused=0, precision="fp32", flow="auto", flow_layers=8, flow_hidden=128,
+                       flow_train_steps=args.flow_steps, flow_batch=256, flow_warmup=args.flow_warmup,
+                       flow_pilot=args.flow_pilot)
+    wall = time.time() - t0
+    st = leg_stats(fit, D)
+    ok = st["rhat_max"] < 1.01
+    ess_s = st["min_bulk_ess"] / fit.sampling_seconds
+    total_s = fit.sampling_seconds + fit.warmup_seconds
+    plain_ess_s = plain["min_bulk_ess"] / plain_samp_s
+    # error of the FP32 contraction against FP64 at the sampler's own draws
+    q = np.stack([fit.draws[k][-1, 7:7 + D] for k in range(0, len(fit.draws), max(1, len(fit.draws) // 64))])
+    lp64, g64 = model.log_prob_grad(q)
+    lp32, g32 = model.log_prob_grad(q, precision="fp32")
+    roof = None
+    if fit.grad_seconds_per_launch > 0 and fit.grad_chains_per_launch > 0 and not fit.flow_used:
+        t = fit.grad_seconds_per_launch
+        ach = 4.0 * args.N * D * fit.grad_chains_per_launch / t / 1e12
+        roof = dict(bound="mfma", achieved=ach, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS,
+                    kernel={0: "glm_mfma_f32", 1: "gl_eta_kernel + gl_grad_kernel",
+                            2: "glm_mfma_bf16"}.get(fit.tuned_path), launch_ms=t * 1e3,
+                    mean_active_chains=fit.grad_chains_per_launch,
+                    note="FP32-level products; peak is the FP32 matrix peak (the BF16-split kernel runs on the BF16 "
+                         "matrix cores, 6 products per FP32 multiply-add)")
+    return dict(
+        path="FP32 contraction (X FP32) + affine-coupling flow (8 layers, MLP 2x128) in automatic mode",
+        autotuned_gradient=["single-pass", "two-pass", "single-pass bf16-split"][fit.tuned_path]
+        if fit.tuned_path >= 0 else None, roofline=roof,
+        flow_used=fit.flow_used, value=ess_s if ok else None,
+        value_null_reason=None if ok else f"not converged: max R-hat {st['rhat_max']:.4f} >= 1.01",
+        min_bulk_ess=st["min_bulk_ess"], rhat_max=st["rhat_max"],
+        ess_per_sec_sampling=ess_s, ess_per_sec_total=st["min_bulk_ess"] / total_s,
+        sampling_seconds=fit.sampling_seconds, warmup_seconds=fit.warmup_seconds, wall_seconds=wall,
+        grad_evals_per_sec=st["grad_evals"] / fit.sampling_seconds,
+        ess_per_grad=st["min_bulk_ess"] / st["grad_evals"],
+        pilot_ess_per_grad_plain=fit.pilot_ess_per_grad_plain, pilot_ess_per_grad_flow=fit.pilot_ess_per_grad_flow,
+        pilot_ess_per_sec_plain=fit.pilot_ess_per_sec_plain, pilot_ess_per_sec_flow=fit.pilot_ess_per_sec_flow,
+        flow_train_seconds=fit.flow_train_seconds, energy_error=st.get("energy_error"),
+        ratio_vs_plain_sampling=ess_s / plain_ess_s,
+        ratio_vs_plain_total=(st["min_bulk_ess"] / total_s) /
+                             (plain["min_bulk_ess"] / (plain_samp_s + args._plain_warm_s)),
+        fp32_error_vs_fp64=dict(points=int(len(q)),
+                                lp_abs_max=float(np.max(np.abs(lp32 - lp64))),
+                                lp_rel_max=float(np.max(np.abs(lp32 - lp64) / np.maximum(1.0, np.abs(lp64)))),
+                                grad_rel_max=float(np.max(np.abs(g32 - g64) /
+                                                          np.maximum(1.0, np.abs(g64).max(axis=1, keepdims=True))))))
 
 
 def bench_c1(args):
@@ -311,6 +424,8 @@ def main():
     ap.add_argument("--D", type=int, default=100)
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned-leg", action="store_true",
+                    help="c2: skip the batched FP32 + flow leg of the metric")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"],
                     help="GLM contraction precision (sums, H and tree arithmetic stay FP64)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
@@ -348,27 +463,25 @@ def main():
         return
 
     import cmdstan_amd as ca
-    from cmdstan_amd.summary import summarize
 
     data, _ = make_data(args.N, args.D, args.seed)
     model = ca.Model("logistic_regression", data)
+    S = step_iterations(args.steps)
+    n_draws = args.steps * S
+    n_warm = max(1000, args.warmup * S)
+    C, D = args.chains, args.D
+    print(f"c2 plain {args.precision}: {n_warm} warmup + {n_draws} draws ({args.steps} steps of {S} iterations)",
+          file=sys.stderr, flush=True)
     t0 = time.time()
-    fit = model.sample(num_chains=args.chains, num_warmup=args.warmup, num_samples=args.steps, seed=args.seed,
+    fit = model.sample(num_chains=C, num_warmup=n_warm, num_samples=n_draws, seed=args.seed,
                        metric="diag_e", fused=0, precision=args.precision)
     wall = time.time() - t0
-    C = args.chains
-    D = args.D
-    # ESS over coefficients and lp__: [params][chains][draws]
-    arr = np.stack([d for d in fit.draws])  # chains x draws x cols
-    params = np.concatenate([arr[:, :, 7:7 + D], arr[:, :, :1]], axis=2).transpose(2, 0, 1)
-    stats = summarize(params)
-    ess_bulk = stats[:, 7]
-    min_ess = float(np.nanmin(ess_bulk))
-    rhat_max = float(np.nanmax(stats[:, 9]))
+    plain = leg_stats(fit, D)
     samp_s = fit.sampling_seconds
-    nlf_samp = float(arr[:, :, 4].sum())
-    grad_per_s = nlf_samp / samp_s
-    value = min_ess / samp_s
+    value = plain["min_bulk_ess"] / samp_s
+    reason = None
+    if not plain["rhat_max"] < 1.01:
+        value, reason = None, f"not converged: max R-hat {plain['rhat_max']:.4f} >= 1.01"
     # roofline of the gradient's contraction kernel: 4 N D flops per active chain
     # per launch (X.B and X^T.R), averaged over every lockstep launch of the run
     roof = None
@@ -378,18 +491,22 @@ def main():
         ach = flops_per_launch / t / 1e12
         peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
         kname = ("glm_mfma_f64" if args.precision == "fp64" else
-                 "glm_mfma_f32" if fit.tuned_path == 0 else "gl_eta_kernel + gl_grad_kernel")
+                 {0: "glm_mfma_f32", 1: "gl_eta_kernel + gl_grad_kernel", 2: "glm_mfma_bf16"}.get(fit.tuned_path))
         roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
                     traffic=measured_traffic(kname, args), kernel=kname, launch_ms=t * 1e3,
                     launches=fit.grad_launches_timed, mean_active_chains=fit.grad_chains_per_launch,
                     flops_per_launch=flops_per_launch)
+    learned = None
+    if not args.no_learned_leg:
+        args._plain_warm_s = fit.warmup_seconds
+        learned = learned_leg(model, args, n_draws, plain, samp_s)
     cpu = None
-    if not args.no_cpu_baseline:
-        ess_per_grad = min_ess / nlf_samp
-        cpu = cpu_baseline(data, fit, ess_per_grad)
+    if not args.no_cpu_baseline and reason is None:
+        cpu = cpu_baseline(data, fit, plain["min_bulk_ess"] / plain["grad_evals"])
     line = {
         "metric": "min bulk-ESS/sec at 1 GPU, batched+learned vs plain NUTS; gradient evals/sec",
         "value": value,
+        "value_null_reason": reason,
         "unit": "ESS/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -401,22 +518,26 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else f"f32 contraction (X {args.precision}), f64 sums",
         "data": "synthetic",
         "config": {"workload": f"{'plain FP64' if args.precision == 'fp64' else 'batched ' + args.precision} NUTS, "
-                               f"Bayesian logistic regression N={args.N} D={D}, {C} chains, diag_e",
-                   "chains": C, "N": args.N, "D": D, "max_depth": 10, "path": "lockstep + batched gradient kernel"},
-        "grad_evals_per_sec": grad_per_s,
-        "min_bulk_ess": min_ess,
-        "rhat_max": rhat_max,
+                               f"Bayesian logistic regression N={args.N} D={D}, {C} chains, diag_e, "
+                               f"{n_warm} warmup + {n_draws} draws",
+                   "chains": C, "N": args.N, "D": D, "max_depth": 10, "path": "lockstep + batched gradient kernel",
+                   "iterations_per_step": S, "warmup_iterations": n_warm, "draws_per_chain": n_draws},
+        "grad_evals_per_sec": plain["grad_evals"] / samp_s,
+        "min_bulk_ess": plain["min_bulk_ess"],
+        "rhat_max": plain["rhat_max"],
         "sampling_seconds": samp_s,
         "warmup_seconds": fit.warmup_seconds,
         "total_wall_seconds": wall,
-        "mean_n_leapfrog": nlf_samp / (C * args.steps),
-        "autotuned_gradient": {"path": ["single-pass", "two-pass"][fit.tuned_path] if fit.tuned_path >= 0 else None,
+        "mean_n_leapfrog": plain["grad_evals"] / (C * n_draws),
+        "energy_error": plain.get("energy_error"),
+        "autotuned_gradient": {"path": ["single-pass", "two-pass", "single-pass bf16-split"][fit.tuned_path]
+                               if fit.tuned_path >= 0 else None,
                                "blocks_per_cu": fit.tuned_blocks_per_cu},
         "roofline": roof,
+        "batched_learned": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
-
 
 if __name__ == "__main__":
     main()

@@ -13,6 +13,9 @@ DP = C.POINTER(C.c_double)
 I64P = C.POINTER(C.c_int64)
 
 
+INTERRUPT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
 class nuts_config(C.Structure):
     _fields_ = [
         ("num_samples", C.c_int), ("num_warmup", C.c_int), ("save_warmup", C.c_int), ("thin", C.c_int),
@@ -23,7 +26,7 @@ class nuts_config(C.Structure):
         ("fused", C.c_int), ("precision", C.c_int), ("max_saved_params", C.c_int),
         ("flow", C.c_int), ("flow_layers", C.c_int), ("flow_hidden", C.c_int), ("flow_train_steps", C.c_int),
         ("flow_batch", C.c_int), ("flow_lr", C.c_double), ("flow_warmup", C.c_int), ("flow_pilot", C.c_int),
-        ("refresh", C.c_int),
+        ("refresh", C.c_int), ("interrupt", INTERRUPT_FN), ("interrupt_user", C.c_void_p),
     ]
 
 
@@ -38,6 +41,21 @@ class nuts_output(C.Structure):
         ("pilot_ess_per_sec_plain", C.c_double), ("pilot_ess_per_sec_flow", C.c_double),
         ("pilot_rhat_plain", C.c_double), ("pilot_rhat_flow", C.c_double),
         ("tuned_path", C.c_int), ("tuned_blocks_per_cu", C.c_int), ("diagnostics", DP),
+        ("energy_error", DP), ("init_rejections", C.POINTER(C.c_int)),
+    ]
+
+
+PLUGIN_GRAD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                          C.c_void_p, C.c_void_p, C.c_void_p)
+PLUGIN_WRITE = C.CFUNCTYPE(C.c_int, C.c_void_p, DP, C.c_int, DP)
+PLUGIN_INITS = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, DP)
+
+
+class nuts_model_plugin(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("num_outputs", C.c_int), ("param_names", C.POINTER(C.c_char_p)),
+        ("unconstrained_names", C.POINTER(C.c_char_p)), ("user", C.c_void_p),
+        ("log_prob_grad", PLUGIN_GRAD), ("write_array", PLUGIN_WRITE), ("transform_inits", PLUGIN_INITS),
     ]
 
 
@@ -47,8 +65,10 @@ SIGNATURES = {
     "nuts_data_destroy": (None, [C.c_void_p]),
     "nuts_data_add": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, I64P, DP, C.c_int]),
     "nuts_data_parse_json": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "nuts_data_parse_rdump": (C.c_int, [C.c_void_p, C.c_char_p]),
     "nuts_data_get": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), I64P, C.POINTER(DP)]),
     "nuts_model_create": (C.c_void_p, [C.c_char_p, C.POINTER(C.c_void_p), C.c_int]),
+    "nuts_model_create_plugin": (C.c_void_p, [C.POINTER(nuts_model_plugin)]),
     "nuts_model_destroy": (None, [C.c_void_p]),
     "nuts_model_num_datasets": (C.c_int, [C.c_void_p]),
     "nuts_model_dims": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -65,6 +85,9 @@ SIGNATURES = {
                                          DP, DP, C.c_int64, DP, C.c_int64, DP, DP]),
     "nuts_transition_replay_precision": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                                    C.c_int, DP, DP, DP, C.c_int64, DP, C.c_int64, C.c_int, DP, DP]),
+    "nuts_transition_replay_driver": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                C.c_int, DP, DP, DP, C.c_int64, DP, C.c_int64, C.c_int, C.c_int, DP,
+                                                DP]),
     "nuts_summary": (C.c_int, [DP, C.c_int, C.c_int, C.c_int, C.c_int, DP]),
     "nuts_flow_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
     "nuts_flow_destroy": (None, [C.c_void_p]),

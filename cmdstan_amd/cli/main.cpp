@@ -13,6 +13,7 @@
 //   config header / CSV layout ... write_config.hpp:19-29; example_output/bernoulli_chain_1.csv
 //   save_metric JSON ............. command.hpp:249-264
 // Exit codes follow return_codes.hpp:21-23 (0 OK, 1 NOT_OK).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -31,6 +32,7 @@
 #include <vector>
 
 #include "../../include/cmdstan_amd.h"
+#include "stan_csv.h"
 
 namespace {
 
@@ -61,6 +63,10 @@ std::unique_ptr<Node> make_tree() {
   Node* method = root->add("method", Node::LIST, "sample");
   method->choices = {"sample", "optimize", "variational", "diagnose", "generate_quantities", "pathfinder",
                      "log_prob", "laplace"};
+  Node* lp = method->add("log_prob", Node::GROUP);  // arg_log_prob.hpp
+  lp->add("unconstrained_params", Node::VALUE, "");
+  lp->add("constrained_params", Node::VALUE, "");
+  lp->add("jacobian", Node::VALUE, "true");
   Node* sample = method->add("sample", Node::GROUP);
   sample->add("num_samples", Node::VALUE, "1000");
   sample->add("num_warmup", Node::VALUE, "1000");
@@ -268,6 +274,179 @@ void fmt_row(std::ostream& o, const double* v, int n) {
   o << "\n";
 }
 
+// get_var_context (command_helper.hpp:116-133): a .json file is JSON, any other
+// file is read as R dump, with CmdStan's deprecation warning.
+void load_var_context(const std::string& path, nuts_data* d, const std::string& what) {
+  const std::string text = read_file(path);
+  const size_t dot = path.find_last_of('.');
+  const bool json = dot != std::string::npos && path.substr(dot) == ".json";
+  int rc;
+  if (json) {
+    rc = nuts_data_parse_json(d, text.c_str());
+  } else {
+    std::cerr << "Warning: file '" << path << "' is being read as an 'RDump' file.\n"
+              << "\tThis format is deprecated and will not receive new features.\n"
+              << "\tConsider saving your data in JSON format instead." << std::endl;
+    rc = nuts_data_parse_rdump(d, text.c_str());
+  }
+  if (rc != NUTS_OK) throw std::invalid_argument("Error reading " + what + " file: " + nuts_last_error());
+}
+
+std::string file_header(const std::string& family) {
+  std::ostringstream f;
+  f << "# stan_version_major = 2\n# stan_version_minor = 36\n# stan_version_patch = 0\n";
+  f << "# model = " << family << "_model\n";
+  std::time_t now = std::time(nullptr);
+  char tb[64];
+  std::strftime(tb, sizeof tb, "%Y-%m-%d %H:%M:%S UTC", std::gmtime(&now));
+  f << "# start_datetime = " << tb << "\n";
+  return f.str();
+}
+
+// Constrained parameter values of one Stan CSV row to a var_context: columns
+// "name" or "name.i.j..." (Stan CSV order) become arrays in row-major order.
+nuts_data* row_context(const std::vector<std::string>& cols, const std::vector<double>& row,
+                       const std::vector<std::string>& wanted) {
+  struct Arr { std::vector<std::vector<int64_t>> idx; std::vector<double> v; };
+  std::map<std::string, Arr> by;
+  for (size_t c = 0; c < cols.size(); ++c) {
+    const std::string& n = cols[c];
+    const size_t dot = n.find('.');
+    const std::string base = n.substr(0, dot);
+    if (std::find(wanted.begin(), wanted.end(), base) == wanted.end()) continue;
+    std::vector<int64_t> ix;
+    if (dot != std::string::npos) {
+      std::stringstream ss(n.substr(dot + 1));
+      std::string t;
+      while (std::getline(ss, t, '.')) ix.push_back(std::atoll(t.c_str()) - 1);
+    }
+    by[base].idx.push_back(ix);
+    by[base].v.push_back(row[c]);
+  }
+  nuts_data* d = nuts_data_create();
+  for (auto& kv : by) {
+    const Arr& a = kv.second;
+    const size_t nd = a.idx[0].size();
+    std::vector<int64_t> dims(nd, 0);
+    for (auto& ix : a.idx) {
+      if (ix.size() != nd) throw std::invalid_argument("constrained_params: inconsistent columns for " + kv.first);
+      for (size_t k = 0; k < nd; ++k) dims[k] = std::max(dims[k], ix[k] + 1);
+    }
+    int64_t n = 1;
+    for (auto x : dims) n *= x;
+    if (n != (int64_t)a.v.size()) throw std::invalid_argument("constrained_params: missing columns for " + kv.first);
+    std::vector<double> vals((size_t)n);
+    for (size_t e = 0; e < a.v.size(); ++e) {
+      int64_t off = 0;
+      for (size_t k = 0; k < nd; ++k) off = off * dims[k] + a.idx[e][k];
+      vals[(size_t)off] = a.v[e];
+    }
+    nuts_data_add(d, kv.first.c_str(), (int)nd, dims.data(), vals.data(), 0);
+  }
+  return d;
+}
+
+// method=log_prob (command.hpp:414-458, services_log_prob_grad at
+// command_helper.hpp:569-600): lp__ and g_<unconstrained name> per point
+int run_log_prob(const Node* R, const std::string& family, const std::string& cfgtxt) {
+  const std::vector<std::string> L = {"method", "log_prob"};
+  auto lv = [&](const std::string& k) { std::vector<std::string> p = L; p.push_back(k); return val(R, p); };
+  const std::string upars = lv("unconstrained_params"), cpars = lv("constrained_params");
+  const bool jacobian = to_bool(lv("jacobian"), "jacobian");
+  const std::string out_file = val(R, {"output", "file"});
+  const int sig_figs = (int)to_int(val(R, {"output", "sig_figs"}), "sig_figs");
+  if (sig_figs < -1 || sig_figs > 18) throw std::invalid_argument("sig_figs must be in [-1, 18]");
+  if (!cpars.empty() && cpars == out_file)
+    throw std::invalid_argument("Filename conflict, constrained_params file " + cpars +
+                                " and output file names are identical, must be different.");
+  if (upars.empty() && cpars.empty())
+    throw std::invalid_argument("No input parameter files provided, cannot calculate log probability density.");
+  if (!upars.empty() && !cpars.empty())
+    throw std::invalid_argument("Cannot specify both input files of both constrained and unconstrained parameter "
+                                "values.");
+  std::cout << cfgtxt << std::endl;
+  nuts_data* data = nuts_data_create();
+  const std::string data_file = val(R, {"data", "file"});
+  if (!data_file.empty()) load_var_context(data_file, data, "data");
+  const nuts_data* dl[1] = {data};
+  nuts_model* model = nuts_model_create(family.c_str(), dl, 1);
+  if (!model) throw std::invalid_argument(nuts_last_error());
+  int D = 0, nout = 0;
+  nuts_model_dims(model, 0, &D, &nout);
+  char buf[256];
+  std::vector<std::string> un, pn;
+  for (int i = 0; i < D; ++i) { nuts_model_unconstrained_name(model, 0, i, buf, sizeof buf); un.push_back(buf); }
+  for (int i = 0; i < nout; ++i) {
+    nuts_model_param_name(model, 0, i, buf, sizeof buf);
+    const std::string n = buf;
+    const std::string base = n.substr(0, n.find('.'));
+    if (std::find(pn.begin(), pn.end(), base) == pn.end()) pn.push_back(base);
+  }
+  std::vector<double> q;  // [points][D]
+  if (!upars.empty()) {
+    // get_uparams_r (command_helper.hpp:520-555): params_r, a vector or an array of vectors
+    nuts_data* u = nuts_data_create();
+    load_var_context(upars, u, "unconstrained parameters");
+    int nd = 0;
+    int64_t dims[8];
+    const double* v = nullptr;
+    if (nuts_data_get(u, "params_r", &nd, dims, &v) != NUTS_OK || nd < 1 || dims[0] == 0)
+      throw std::invalid_argument("Unconstrained parameters file has no variable 'params_r' with unconstrained "
+                                  "parameter values!");
+    const int64_t rows = nd == 2 ? dims[0] : 1, cols = nd == 2 ? dims[1] : dims[0];
+    if (cols != D)
+      throw std::invalid_argument("Incorrect number of unconstrained parameters provided! Model has " +
+                                  std::to_string(D) + " parameters but " + std::to_string(cols) + " were found.");
+    q.assign(v, v + rows * cols);
+    nuts_data_destroy(u);
+  } else {
+    const size_t dot = cpars.find_last_of('.');
+    const bool csv = dot != std::string::npos && cpars.substr(dot) == ".csv";
+    std::vector<nuts_data*> ctx;
+    if (csv) {
+      const stancsv::ChainFile f = stancsv::read_csv(cpars);
+      for (const auto& row : f.rows) ctx.push_back(row_context(f.names, row, pn));
+      if (ctx.empty()) throw std::invalid_argument("constrained_params: no draws in " + cpars);
+    } else {
+      nuts_data* c = nuts_data_create();
+      load_var_context(cpars, c, "constrained parameters");
+      ctx.push_back(c);
+    }
+    q.resize(ctx.size() * (size_t)D);
+    for (size_t k = 0; k < ctx.size(); ++k) {
+      const int rc = nuts_model_transform_inits(model, 0, ctx[k], q.data() + k * D);
+      nuts_data_destroy(ctx[k]);
+      if (rc != NUTS_OK) throw std::invalid_argument(nuts_last_error());
+      for (int i = 0; i < D; ++i)
+        if (std::isnan(q[k * D + i]))
+          throw std::invalid_argument("constrained_params: no value for parameter " + un[i]);
+    }
+  }
+  const int n = (int)(q.size() / D);
+  std::vector<double> lp(n), g((size_t)n * D);
+  std::ofstream f(out_file);
+  if (!f) throw std::invalid_argument("Can't open output file " + out_file);
+  f << file_header(family);
+  std::istringstream ct(cfgtxt);
+  std::string line;
+  while (std::getline(ct, line)) f << "# " << line << "\n";
+  if (nuts_log_prob_grad(model, 0, q.data(), n, jacobian ? 1 : 0, lp.data(), g.data()) != NUTS_OK) {
+    std::cerr << "Error during log_prob calculation:\n" << nuts_last_error() << std::endl;
+    return 1;
+  }
+  f << std::setprecision(sig_figs < 0 ? 6 : sig_figs) << "lp__";
+  for (auto& nm : un) f << ",g_" << nm;
+  f << "\n";
+  for (int k = 0; k < n; ++k) {
+    f << lp[k];
+    for (int i = 0; i < D; ++i) f << "," << g[(size_t)k * D + i];
+    f << "\n";
+  }
+  nuts_model_destroy(model);
+  nuts_data_destroy(data);
+  return 0;
+}
+
 int usage(const std::string& exe) {
   std::cout << "Usage: " << exe << " <arg1> <subarg1_1> ... <subarg1_m> ... <arg_n> <subarg_n_1> ... <subarg_n_m>\n\n"
             << "Begin by selecting amongst the following inference methods and diagnostics,\n"
@@ -294,8 +473,14 @@ int run(int argc, char** argv) {
   parse(tree.get(), toks);
   const Node* R = tree.get();
   const std::string method = val(R, {"method"});
+  if (method == "log_prob") {
+    std::ostringstream lpt;
+    print_tree(R, 0, lpt, "");
+    return run_log_prob(R, family, lpt.str());
+  }
   if (method != "sample")
-    throw std::invalid_argument("method=" + method + " is not provided by this build; only method=sample runs here");
+    throw std::invalid_argument("method=" + method + " is not provided by this build; method=sample and "
+                                "method=log_prob run here");
   const std::vector<std::string> S = {"method", "sample"};
   auto sv = [&](std::vector<std::string> p) { std::vector<std::string> q = S; q.insert(q.end(), p.begin(), p.end()); return val(R, q); };
   if (sv({"algorithm"}) != "hmc")
@@ -351,7 +536,8 @@ int run(int argc, char** argv) {
   if (refresh < 0) throw std::invalid_argument("refresh must be >= 0");
   cfg.refresh = (int)refresh;
   if (save_metric && !cfg.adapt_engaged)
-    throw std::invalid_argument("Cannot save adapted metric if adaptation is disabled.");  // command.hpp:249-259
+    throw std::invalid_argument("Saving the adapted metric (save_metric=1) can only be enabled when adaptation is "
+                                "engaged (adapt engaged=1)");  // command.hpp:249-259
   if (nuts_config_validate(&cfg) != NUTS_OK) throw std::invalid_argument(nuts_last_error());
 
   const std::string out_file = val(R, {"output", "file"});
@@ -384,8 +570,7 @@ int run(int argc, char** argv) {
   }
 
   nuts_data* data = nuts_data_create();
-  if (!data_file.empty() && nuts_data_parse_json(data, read_file(data_file).c_str()) != NUTS_OK)
-    throw std::invalid_argument(std::string("Error reading data file: ") + nuts_last_error());
+  if (!data_file.empty()) load_var_context(data_file, data, "data");
   const nuts_data* dl[1] = {data};
   nuts_model* model = nuts_model_create(family.c_str(), dl, 1);
   if (!model) throw std::invalid_argument(nuts_last_error());
@@ -398,8 +583,7 @@ int run(int argc, char** argv) {
   std::vector<double> inv_metric;
   if (!metric_file.empty()) {
     nuts_data* md = nuts_data_create();
-    if (nuts_data_parse_json(md, read_file(metric_file).c_str()) != NUTS_OK)
-      throw std::invalid_argument(std::string("Error reading metric file: ") + nuts_last_error());
+    load_var_context(metric_file, md, "metric");
     int nd = 0;
     int64_t dims[8];
     const double* v = nullptr;
@@ -436,33 +620,62 @@ int run(int argc, char** argv) {
   // the named file for every chain (command_helper.hpp:144-245)
   std::vector<double> inits;
   if (init_is_file) {
-    const size_t dot = init.find_last_of('.');
-    if (dot == std::string::npos || init.substr(dot) != ".json")
-      throw std::invalid_argument("Found: \"" + init + "\" but user specified files must end in .json");
+    // get_vec_var_context (command_helper.hpp:144-245): one chain reads the file
+    // as given; several chains need a .json or .R ending and use per-chain files
+    // {name}_{id+i}{ext} when the first exists, else the named file for all
     std::vector<std::string> files(cfg.num_chains, init);
     if (cfg.num_chains > 1) {
-      const auto per = make_filenames(init.substr(0, dot), "", ".json", cfg.num_chains, cfg.id);
-      if (std::ifstream(per[0]).good()) files = per;
-      else if (!std::ifstream(init).good())
-        throw std::invalid_argument("Can't open either of specified files, \"" + per[0] + "\" and base file \"" +
+      const size_t dot = init.find_last_of('.');
+      if (dot == std::string::npos)
+        throw std::invalid_argument("Found: \"" + init + "\" but user specified files must end in .json or .R");
+      const std::string ext = init.substr(dot);
+      if (ext != ".json" && ext != ".R")
+        throw std::invalid_argument("file ending of " + ext + " is not supported by cmdstan");
+      const auto per = make_filenames(init.substr(0, dot), "", ext, cfg.num_chains, cfg.id);
+      if (std::ifstream(per[0]).good()) {
+        files = per;
+        for (int c = 1; c < cfg.num_chains; ++c)
+          if (!std::ifstream(per[c]).good())
+            throw std::invalid_argument("Found \"" + per[0] + "\" but cannot open \"" + per[c] + "\"");
+      } else if (!std::ifstream(init).good()) {
+        throw std::invalid_argument("Can't open either of specified files,\"" + per[0] + "\" and base file \"" +
                                     init + "\"");
+      }
     }
     inits.resize((size_t)cfg.num_chains * D);
     for (int c = 0; c < cfg.num_chains; ++c) {
       if (!std::ifstream(files[c]).good()) throw std::invalid_argument("Can't open init file " + files[c]);
       nuts_data* idata = nuts_data_create();
-      const bool ok = nuts_data_parse_json(idata, read_file(files[c]).c_str()) == NUTS_OK &&
-                      nuts_model_transform_inits(model, 0, idata, inits.data() + (size_t)c * D) == NUTS_OK;
+      load_var_context(files[c], idata, "init");
+      const bool ok = nuts_model_transform_inits(model, 0, idata, inits.data() + (size_t)c * D) == NUTS_OK;
       nuts_data_destroy(idata);
       if (!ok) throw std::invalid_argument(nuts_last_error());
     }
   }
+  std::vector<int> rejections((size_t)C * 2, 0);
+  out.init_rejections = rejections.data();
   const int rc = nuts_sample(model, &cfg, inits.empty() ? nullptr : inits.data(),
                              inv_metric.empty() ? nullptr : inv_metric.data(), &out);
+  // services::util::initialize's messages, one block per rejected initial value
+  for (int k = 0; k < C; ++k) {
+    const std::string pre = C > 1 ? "Chain [" + std::to_string(cfg.id + k) + "] " : "";
+    for (int r = 0; r < rejections[2 * k]; ++r)
+      std::cout << pre << "Rejecting initial value:\n" << pre
+                << "  Log probability evaluates to log(0), i.e. negative infinity.\n" << pre
+                << "  Stan can't start sampling from this initial value.\n";
+    for (int r = 0; r < rejections[2 * k + 1]; ++r)
+      std::cout << pre << "Rejecting initial value:\n" << pre
+                << "  Gradient evaluated at the initial value is not finite.\n" << pre
+                << "  Stan can't start sampling from this initial value.\n";
+    if (status[k] == 1 && !init_is_file && cfg.init_radius > 0)
+      std::cout << pre << "\n" << pre << "Initialization between (-" << cfg.init_radius << ", " << cfg.init_radius
+                << ") failed after 100 attempts. \n" << pre
+                << " Try specifying initial values, reducing ranges of constrained values, or reparameterizing the "
+                   "model.\n";
+  }
+  std::cout << std::flush;
   if (rc != NUTS_OK) {
     std::cerr << nuts_last_error() << std::endl;
-    for (int k = 0; k < C; ++k)
-      if (status[k] == 1) std::cerr << "Chain [" << (cfg.id + k) << "] Rejecting initial value: initialization failed." << std::endl;
     return 1;
   }
 

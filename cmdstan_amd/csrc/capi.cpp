@@ -19,6 +19,7 @@
 
 #include "../../include/cmdstan_amd.h"
 #include "json.h"
+#include "rdump.h"
 #include "launch.h"
 #include "flow.h"
 #include "hier_logistic.h"
@@ -133,11 +134,27 @@ struct ModelImpl {
   const double* d_X = nullptr;
   const double* d_y = nullptr;
   // autotuned GLM gradient per (chains, precision): path 0 = single-pass
-  // kernel with blocks_per_cu resident blocks per CU, path 1 = two-pass kernels
+  // kernel with blocks_per_cu resident blocks per CU, path 1 = two-pass kernels,
+  // path 2 = single-pass BF16-split kernel (FP32-level accuracy on BF16 matrix cores)
   struct GlmTune { int path = 0, blocks_per_cu = 2; double ms = 0; };
   std::map<std::pair<int, int>, GlmTune> glm_tune;
+  // caller-supplied model (nuts_model_create_plugin)
+  bool plugin = false;
+  nuts_model_plugin pl{};
   DevBuf<unsigned char> xlow;  // X converted to FP32 or BF16 (lazily)
   int xlow_prec = 0;
+  DevBuf<unsigned short> xbf;  // X as BF16 terms for the BF16-split kernel (lazily)
+  int xbf_prec = 0;
+  const unsigned short* bf16_x(int precision, hipStream_t st) {
+    if (xbf_prec != precision) {
+      const int64_t n = (int64_t)glmN * glm_bf16_ldx(glmD);
+      xbf.alloc((size_t)n * (precision == 1 ? 3 : 1));
+      glm_bf16_prepare_x(d_X, glmN, glmD, precision, xbf.p, st);
+      HIPC(hipStreamSynchronize(st));
+      xbf_prec = precision;
+    }
+    return xbf.p;
+  }
   const void* low_x(int precision, hipStream_t st) {
     if (xlow_prec != precision) {
       const int64_t n = (int64_t)glmN * glmD;
@@ -319,6 +336,8 @@ struct RunSpec {
   FlowNet* flow = nullptr;                      // sample z with x = T(z) (single dataset, lockstep)
   const double* stepsize_per_chain = nullptr;   // [C] resume with these step sizes (no adaptation)
   bool diagnostics = false;                     // record q, p, g of every saved draw
+  bool energy_error = false;                    // record each saved row's max |H - H0|
+  int block_mode = -1;                          // lockstep state machine: -1 by dimension, 0 wave, 1 block per chain
 };
 
 struct RunResult {
@@ -336,6 +355,7 @@ struct RunResult {
   std::vector<double> z_last;  // [C][D] unconstrained q of each chain's last saved row (before any flow map)
   int rows = 0;
   std::vector<double> diag;    // diagnostics: chains back to back, rows x (7 + 3 D)
+  std::vector<double> eerr;    // energy error: [C][rows]
 };
 
 int output_rows(const nuts_config& c) {
@@ -433,6 +453,14 @@ ModelImpl::GlmTune glm_autotune(ModelImpl& M, int C, int precision, hipStream_t 
     if (ms < best.ms) { best.ms = ms; best.path = 0; best.blocks_per_cu = bpc; }
   }
   if (precision != 0) {
+    const unsigned short* xb = M.bf16_x(precision, st);
+    for (int bpc = 1; bpc <= 4; ++bpc) {
+      const GlmPlan gp = glm_mfma_plan(N, C, D, num_cus(), bpc);
+      DevBuf<double> work(gp.work_doubles);
+      GlmArgs ga{M.d_X, M.d_y, N, D, C, gp.rows_per_chunk, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
+      const double ms = time_it([&]() { glm_grad_mfma_bf16(ga, gp, xb, precision, st); });
+      if (ms < best.ms) { best.ms = ms; best.path = 2; best.blocks_per_cu = bpc; }
+    }
     const GlLargePlan gp = gl_large_plan(N, C, D, num_cus());
     DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
     DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
@@ -445,6 +473,12 @@ ModelImpl::GlmTune glm_autotune(ModelImpl& M, int C, int precision, hipStream_t 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIPC(hipStreamSynchronize(st));
+  // CMDSTAN_AMD_GLM_PATH=0|1|2 forces a reduced-precision path (tests compare them)
+  if (const char* fp = std::getenv("CMDSTAN_AMD_GLM_PATH"))
+    if (precision != 0) {
+      const int want = std::atoi(fp);
+      if (want >= 0 && want <= 2 && want != best.path) { best.path = want; best.blocks_per_cu = 2; }
+    }
   M.glm_tune[key] = best;
   return best;
 }
@@ -556,8 +590,10 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         }
     }
     dMb.upload(hm, st);
-    HIPC(hipStreamSynchronize(st));
     dLb.alloc(mtot);
+    // the identity start metric is its own Cholesky factor: no factorisation
+    if (!R.inv_metric) dLb.upload(hm.data(), hm.size(), st);
+    HIPC(hipStreamSynchronize(st));
     dWb.alloc(mtot);
     HIPC(hipMemsetAsync(dWb.p, 0, mtot * sizeof(double), st));
   }
@@ -605,6 +641,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   P.fused_budget = 256;
   P.save_params = msp;
   P.diag = R.diagnostics ? 1 : 0;
+  P.out_rows = rows;
 
   NutsBuffers B;
   std::memset(&B, 0, sizeof(B));
@@ -616,7 +653,12 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   B.n_inj_u = R.n_u; B.n_inj_z = R.n_z;
   B.dM = dMb.p; B.dL = dLb.p; B.dW = dWb.p;
   B.diag = ddiag.p; B.diag_off = ddiag_off.p;
-  if (dense) {
+  DevBuf<double> deerr;
+  if (R.energy_error && rows > 0) {
+    deerr.alloc((size_t)C * rows);
+    B.eerr = deerr.p;
+  }
+  if (dense && R.inv_metric) {
     DevBuf<int> nfail(1);
     HIPC(hipMemsetAsync(nfail.p, 0, sizeof(int), st));
     launch_dense_factor(B, nfail.p, st);
@@ -634,19 +676,22 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   const char* gle = std::getenv("CMDSTAN_AMD_GLM_LARGE");
   ModelImpl::GlmTune tune;
   int res_tuned_path = -1, res_tuned_bpc = 0;
-  if (M.glm && nds == 1 && !R.flow && cfg.fused != 1) tune = glm_autotune(M, C, cfg.precision, st);
+  // the batched GLM gradients also serve the flow (at the rows x = T(z))
+  const bool glm_batch = M.glm && nds == 1 && (cfg.fused != 1 || R.flow);
+  if (glm_batch) tune = glm_autotune(M, C, cfg.precision, st);
   const bool small_as_large =
       M.glm && cfg.precision != 0 && (gle ? std::atoi(gle) != 0 : tune.path == 1);
-  const bool glm_ok = M.glm && nds == 1 && !R.flow && !small_as_large;
-  if (M.glm && nds == 1 && !R.flow && cfg.fused != 1) {
-    res_tuned_path = small_as_large ? 1 : 0;
+  const bool glm_ok = glm_batch && !small_as_large;
+  if (glm_batch) {
+    res_tuned_path = small_as_large ? 1 : tune.path;
     res_tuned_bpc = tune.blocks_per_cu;
   }
   // hierarchical logistic: batched group kernel (lane = chain) over one dataset
   const bool hl_ok = M.kind == MK_HIER_LOGISTIC && nds == 1;
   // large-D logistic regression: two-pass FP32 matrix-core gradient (X FP32 or BF16)
-  const bool gl_ok = (M.glm_large || small_as_large) && nds == 1 && !R.flow && cfg.precision != 0 && cfg.fused != 1;
-  if (cfg.fused < 0) fused = !glm_ok && !hl_ok && !gl_ok;
+  const bool gl_ok = (M.glm_large || small_as_large) && nds == 1 && cfg.precision != 0 && (cfg.fused != 1 || R.flow);
+  if (M.plugin && cfg.fused == 1) throw std::invalid_argument("plugin models run on the lockstep kernels (fused=0)");
+  if (cfg.fused < 0) fused = !glm_ok && !hl_ok && !gl_ok && !M.plugin;
   if (R.flow) {
     if (nds != 1 || R.flow->dim() != D[0] || R.flow->max_rows() < C)
       throw std::invalid_argument("flow: one dataset whose dimension matches the flow, at most max_rows chains");
@@ -670,13 +715,15 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   DevBuf<double> work;
   DevBuf<int> act_idx, n_act;
   const void* xlow = nullptr;
+  const unsigned short* xbf = nullptr;  // BF16 terms of X (path 2)
   GlmArgs ga;
   GlmPlan gplan;
   // large chains: one 256-thread block per chain for the state machine
   int maxD = 0;
   for (int c = 0; c < C; ++c) maxD = std::max(maxD, D[c]);
   const char* bpc = std::getenv("CMDSTAN_AMD_BLOCK_PER_CHAIN");
-  const bool block_per_chain = !fused && (bpc ? std::atoi(bpc) != 0 : maxD >= 512);
+  const bool block_per_chain =
+      !fused && (R.block_mode >= 0 ? R.block_mode == 1 : (bpc ? std::atoi(bpc) != 0 : maxD >= 512));
   DevBuf<double> hlwork;
   HlArgs hla{};
   if (hl_ok && !fused) {
@@ -705,6 +752,18 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
       hla.Q = fX.p; hla.ldq = D[0]; hla.Gout = fGX.p; hla.ldg = D[0]; hla.Vout = fVX.p;
     }
   }
+  // flow: the GLM gradients read the packed rows x = T(z) and write packed
+  // rows; the identity list maps position r to row r
+  DevBuf<int> ident;
+  if (R.flow && (glm_ok || gl_ok)) {
+    std::vector<int> h(C);
+    std::iota(h.begin(), h.end(), 0);
+    ident.upload(h, st);
+  }
+  if (!fused && M.plugin && !act_idx.p) {
+    act_idx.alloc(C);
+    n_act.alloc(1);
+  }
   GlLargeArgs gla;
   GlLargePlan glp;
   DevBuf<float> glQf, glR;
@@ -717,6 +776,9 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     gla.y = M.d_y; gla.N = M.glmN; gla.D = M.glmD; gla.C = C; gla.s2inv = M.glm_s2inv;
     gla.Q = Q.p; gla.ldq = Dp[0]; gla.G = G.p; gla.V = V.p; gla.idx = act_idx.p; gla.n_active = n_act.p;
     gla.Qf = glQf.p; gla.Dq = glp.Dq; gla.R = glR.p; gla.ldr = glp.ldr; gla.Wg = glWg.p; gla.Wlp = glWlp.p;
+    if (R.flow) {
+      gla.Q = fX.p; gla.ldq = D[0]; gla.G = fGX.p; gla.V = fVX.p; gla.idx = ident.p;
+    }
     xlow = M.low_x(cfg.precision, st);
   }
   if (!fused && glm_ok) {
@@ -727,8 +789,21 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     ga.X = M.d_X; ga.y = M.d_y; ga.N = M.glmN; ga.D = M.glmD; ga.C = C; ga.rows_per_chunk = gplan.rows_per_chunk;
     ga.s2inv = M.glm_s2inv; ga.Q = Q.p; ga.ldq = Dp[0]; ga.G = G.p; ga.V = V.p; ga.work = work.p;
     ga.idx = act_idx.p; ga.n_active = n_act.p;
-    if (cfg.precision != 0) xlow = M.low_x(cfg.precision, st);
+    if (R.flow) {
+      ga.Q = fX.p; ga.ldq = D[0]; ga.G = fGX.p; ga.V = fVX.p; ga.idx = ident.p;
+    }
+    if (cfg.precision != 0) {
+      if (tune.path == 2) xbf = M.bf16_x(cfg.precision, st);
+      else xlow = M.low_x(cfg.precision, st);
+    }
   }
+  // the GLM gradient of the chosen kernel (FP64, FP32 or BF16-split)
+  auto glm_grad = [&]() {
+    const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
+                    : xbf ? glm_grad_mfma_bf16(ga, gplan, xbf, cfg.precision, st)
+                          : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
+    if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+  };
 
   RunResult res;
   res.rows = rows;
@@ -765,34 +840,51 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   // One check interval of lockstep work: kCheck x (advance, gradient).  With
   // events, each gradient's contraction (or whole gradient) is bracketed.
   auto enqueue_bundle = [&](bool with_events) {
+    // active chains are tallied over the timed launches only (the roofline pairs them with their times)
+    long long* tal = with_events ? tally.p : nullptr;
     for (int k = 0; k < kCheck; ++k) {
       if (block_per_chain) launch_advance_external_block(cfg.metric, P, B, st);
       else launch_advance_external(cfg.metric, P, B, st);
-      if (glm_ok) {
-        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+      if (R.flow) {
+        // pulled-back gradient: x = T(z) for the active chains, model at x, VJP to z
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tal, st);
+        if (with_events) HIPC(hipEventRecord(ev.a[k], st));
+        R.flow->forward(Q.p, Dp[0], act_idx.p, n_act.p, C, fX.p, fLD.p);
+        if (hl_ok) {
+          hl_grad_launch(hla, hl_groups, st);
+        } else if (glm_ok) {
+          ga.n_active = n_act.p;
+          glm_grad();
+        } else if (gl_ok) {
+          gl_large_grad(gla, glp, xlow, cfg.precision, st);
+        } else if (M.plugin) {
+          if (M.pl.log_prob_grad(M.pl.user, fX.p, D[0], nullptr, n_act.p, C, 1, fGX.p, fVX.p, st))
+            throw std::runtime_error("plugin log_prob_grad failed");
+        } else {
+          launch_model_grad_rows(M.kind, act_idx.p, n_act.p, C, D[0], fX.p, fGX.p, fVX.p, md, st);
+        }
+        R.flow->pullback(fGX.p, fVX.p, fLD.p, act_idx.p, n_act.p, C, G.p, Dp[0], V.p);
+        if (with_events) HIPC(hipEventRecord(ev.b[k], st));
+      } else if (glm_ok) {
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tal, st);
         ga.ev_begin = with_events ? ev.a[k] : nullptr;
         ga.ev_end = with_events ? ev.b[k] : nullptr;
-        const int rcg = cfg.precision == 0 ? glm_grad_mfma_f64(ga, gplan, st)
-                                           : glm_grad_mfma_f32(ga, gplan, xlow, cfg.precision, kF32Layout, st);
-        if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+        glm_grad();
       } else if (gl_ok) {
-        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tal, st);
         gla.ev_begin = with_events ? ev.a[k] : nullptr;
         gla.ev_end = with_events ? ev.b[k] : nullptr;
         gl_large_grad(gla, glp, xlow, cfg.precision, st);
-      } else if (R.flow) {
-        // pulled-back gradient: x = T(z) for the active chains, model at x, VJP to z
-        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
-        if (with_events) HIPC(hipEventRecord(ev.a[k], st));
-        R.flow->forward(Q.p, Dp[0], act_idx.p, n_act.p, C, fX.p, fLD.p);
-        if (hl_ok) hl_grad_launch(hla, hl_groups, st);
-        else launch_model_grad_rows(M.kind, act_idx.p, n_act.p, C, D[0], fX.p, fGX.p, fVX.p, md, st);
-        R.flow->pullback(fGX.p, fVX.p, fLD.p, act_idx.p, n_act.p, C, G.p, Dp[0], V.p);
-        if (with_events) HIPC(hipEventRecord(ev.b[k], st));
       } else if (hl_ok) {
-        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tally.p, st);
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tal, st);
         if (with_events) HIPC(hipEventRecord(ev.a[k], st));
         hl_grad_launch(hla, hl_groups, st);
+        if (with_events) HIPC(hipEventRecord(ev.b[k], st));
+      } else if (M.plugin) {
+        launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tal, st);
+        if (with_events) HIPC(hipEventRecord(ev.a[k], st));
+        if (M.pl.log_prob_grad(M.pl.user, Q.p, Dp[0], act_idx.p, n_act.p, C, 1, G.p, V.p, st))
+          throw std::runtime_error("plugin log_prob_grad failed");
         if (with_events) HIPC(hipEventRecord(ev.b[k], st));
       } else {
         if (with_events) HIPC(hipEventRecord(ev.a[k], st));
@@ -802,11 +894,15 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     }
   };
   // Lockstep launch sequences are replayed from a captured HIP graph (one per
-  // phase: the phase's parameters are baked in).  Default: on for the
-  // launch-bound paths (flow, per-chain models), off for the GLM contraction,
-  // whose kernel is timed with events.
+  // phase: the phase's parameters are baked in); default on, except for
+  // plugin models (their gradient callback is the caller's code).  One bundle
+  // in kTimeEvery is launched eagerly with events around the gradient's
+  // contraction kernel (the whole gradient off the GLM path): the kernel
+  // timing comes from those, outside the captured graph.
   const char* genv = std::getenv("CMDSTAN_AMD_GRAPH");
-  const bool use_graph = !fused && (genv ? std::atoi(genv) != 0 : !glm_ok && !gl_ok);
+  const bool use_graph = !fused && (genv ? std::atoi(genv) != 0 : !M.plugin);
+  constexpr int kTimeEvery = 8;
+  int bundle_no = 0;
   struct GraphHold {
     hipGraph_t g = nullptr;
     hipGraphExec_t x = nullptr;
@@ -822,9 +918,11 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     for (;;) {
       if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() > max_seconds)
         throw std::runtime_error("sampler did not finish within CMDSTAN_AMD_MAX_SECONDS");
+      if (cfg.interrupt && cfg.interrupt(cfg.interrupt_user)) throw std::runtime_error("sampling interrupted");
+      bool timed = false;
       if (fused) {
         launch_advance_fused(M.kind, cfg.metric, vec32, P, B, md, st);
-      } else if (use_graph) {
+      } else if (use_graph && bundle_no++ % kTimeEvery != 0) {
         if (!graph.x) {
           HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
           enqueue_bundle(false);
@@ -836,6 +934,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
       } else {
         enqueue_bundle(true);
         res.leaf_steps += kCheck;
+        timed = true;
       }
       HIPC(hipGetLastError());
       HIPC(hipMemcpyAsync(&stopped, dstop.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -855,7 +954,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
           next_report += cfg.refresh;
         }
       }
-      if (!fused && !use_graph) {
+      if (timed) {
         for (int k = 0; k < kCheck; ++k) {
           float ms = 0;
           HIPC(hipEventElapsedTime(&ms, ev.a[k], ev.b[k]));
@@ -939,7 +1038,31 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
       R.flow->map_rows(draws.p + k0 * ld, ld, n);
     }
   }
-  if (R.constrain && msp < 0) {
+  if (R.constrain && msp < 0 && M.plugin) {
+    // write_array on the host: unconstrained rows down, constrained outputs by callback
+    std::vector<double> un(dtot);
+    HIPC(hipMemcpyAsync(un.data(), draws.p, dtot * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    res.draws.assign(otot, 0.0);
+    std::vector<double> q((size_t)rows * D[0]), o((size_t)rows * nout[0]);
+    for (int c = 0; c < C; ++c) {
+      const double* src = un.data() + doff[c];
+      double* dst = res.draws.data() + ooff[c];
+      for (int r = 0; r < rows; ++r)
+        for (int i = 0; i < D[c]; ++i) q[(size_t)r * D[c] + i] = src[(int64_t)r * (7 + D[c]) + 7 + i];
+      if (M.pl.write_array) {
+        if (M.pl.write_array(M.pl.user, q.data(), rows, o.data())) throw std::runtime_error("plugin write_array failed");
+      } else {
+        o = q;
+      }
+      for (int r = 0; r < rows; ++r) {
+        for (int j = 0; j < 7; ++j) dst[(int64_t)r * (7 + nout[c]) + j] = src[(int64_t)r * (7 + D[c]) + j];
+        for (int j = 0; j < nout[c]; ++j) dst[(int64_t)r * (7 + nout[c]) + 7 + j] = o[(size_t)r * nout[c] + j];
+      }
+    }
+    res.draw_off = ooff;
+    for (int c = 0; c < C; ++c) res.cols.push_back(7 + nout[c]);
+  } else if (R.constrain && msp < 0) {
     DevBuf<double> out(std::max<int64_t>(otot, 1));
     DevBuf<int> dn(C);
     dn.upload(nout, st);
@@ -957,6 +1080,10 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   if (R.diagnostics) {
     res.diag.resize(diag_tot);
     HIPC(hipMemcpyAsync(res.diag.data(), ddiag.p, diag_tot * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  if (B.eerr) {
+    res.eerr.resize((size_t)C * rows);
+    HIPC(hipMemcpyAsync(res.eerr.data(), deerr.p, res.eerr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   }
   HIPC(hipStreamSynchronize(st));
   HIPC(hipGetLastError());
@@ -1004,6 +1131,7 @@ struct FlowReport {
   int used = 0;
   double train_s = 0, loss = 0, egp = 0, egf = 0, esp = 0, esf = 0, rhp = 0, rhf = 0, pre_s = 0;
   std::vector<int64_t> nlf, ngrad;  // per chain, over every phase
+  std::vector<int> rej_lp, rej_grad;  // initial values rejected in the plain phase
 };
 
 // Plain warmup + pilot, flow training, flow warmup + pilot, then the better
@@ -1023,8 +1151,13 @@ RunResult run_with_flow(ModelImpl& M, const RunSpec& R0, FlowReport& rep) {
   RunSpec A = R0;
   A.cfg.flow = 0; A.cfg.save_warmup = 1; A.cfg.thin = 1; A.cfg.num_samples = P; A.cfg.max_saved_params = -1;
   A.constrain = false;
+  A.energy_error = false;
   RunResult ra = run_batch(M, A);
   add_counts(ra);
+  for (int c = 0; c < C; ++c) {
+    rep.rej_lp.push_back(ra.st[c].init_rej_lp);
+    rep.rej_grad.push_back(ra.st[c].init_rej_grad);
+  }
   if (!all_done(ra)) return ra;
   const int rowsA = cfg.num_warmup + P;
   rep.egp = pilot_ess_per_grad(ra, cfg.num_warmup, rowsA, D, &rep.rhp);
@@ -1074,6 +1207,7 @@ RunResult run_with_flow(ModelImpl& M, const RunSpec& R0, FlowReport& rep) {
   Bs.cfg.flow = 0; Bs.cfg.num_warmup = cfg.flow_warmup; Bs.cfg.num_samples = P; Bs.cfg.save_warmup = 0;
   Bs.cfg.thin = 1; Bs.cfg.max_saved_params = -1;
   Bs.init = z0.data(); Bs.inv_metric = nullptr; Bs.flow = &net; Bs.constrain = false;
+  Bs.energy_error = false;
   RunResult rb = run_batch(M, Bs);
   add_counts(rb);
   const bool flow_ok = all_done(rb);
@@ -1173,6 +1307,21 @@ int nuts_data_parse_json(nuts_data* d, const char* text) {
   });
 }
 
+int nuts_data_parse_rdump(nuts_data* d, const char* text) {
+  return guarded([&]() {
+    std::string t(text);
+    RDumpParser p(t);
+    for (auto& v : p.parse()) {
+      Field f;
+      f.dims = v.dims;
+      f.v = std::move(v.vals);
+      f.is_int = v.is_int;
+      d->f[v.name] = std::move(f);
+    }
+    return NUTS_OK;
+  });
+}
+
 int nuts_data_get(const nuts_data* d, const char* name, int* ndim, int64_t* dims, const double** values) {
   return guarded([&]() {
     const Field& f = d->get(name);
@@ -1211,6 +1360,41 @@ nuts_model* nuts_model_create(const char* family, const nuts_data* const* datase
       M.d_X = M.d_data.p + 3;
       M.d_y = M.d_data.p + 3 + (size_t)M.glmN * M.glmD;
     }
+    return m.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+nuts_model* nuts_model_create_plugin(const nuts_model_plugin* p) {
+  try {
+    if (!p || p->dim < 1 || p->num_outputs < 1 || !p->log_prob_grad || !p->param_names || !p->unconstrained_names)
+      throw std::invalid_argument("plugin: dim, num_outputs, names and log_prob_grad are required");
+    if (!p->write_array && p->num_outputs != p->dim)
+      throw std::invalid_argument("plugin: without write_array the outputs are the unconstrained values");
+    std::unique_ptr<nuts_model> m(new nuts_model());
+    ModelImpl& M = m->impl;
+    M.family = "plugin";
+    M.kind = MK_PLUGIN;
+    M.plugin = true;
+    M.pl = *p;
+    Dataset d;
+    d.D = p->dim;
+    d.nout = p->num_outputs;
+    for (int i = 0; i < p->num_outputs; ++i) {
+      if (!p->param_names[i]) throw std::invalid_argument("plugin: null parameter name");
+      d.names.push_back(p->param_names[i]);
+    }
+    for (int i = 0; i < p->dim; ++i) {
+      if (!p->unconstrained_names[i]) throw std::invalid_argument("plugin: null unconstrained name");
+      d.unames.push_back(p->unconstrained_names[i]);
+    }
+    M.pl.param_names = nullptr;  // copied above; the caller may free its arrays
+    M.pl.unconstrained_names = nullptr;
+    M.ds.push_back(std::move(d));
+    M.ds_off.push_back(0);
+    M.d_data.alloc(1);
     return m.release();
   } catch (const std::exception& e) {
     g_err = e.what();
@@ -1266,7 +1450,11 @@ static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n
     ModelDataDev md{M.d_data.p, ddso.p, jacobian ? 1.0 : 0.0};
     if (precision != 0 && !((M.glm || M.glm_large) && dataset == 0))
       throw std::invalid_argument("reduced precision is available for the batched logistic regression only");
-    if (M.glm_large && dataset == 0 && precision != 0) {
+    if (M.plugin) {
+      if (M.pl.log_prob_grad(M.pl.user, Q.p, Dp, nullptr, nullptr, C, jacobian ? 1 : 0, G.p, V.p, nullptr))
+        throw std::runtime_error("plugin log_prob_grad failed");
+      HIPC(hipDeviceSynchronize());
+    } else if (M.glm_large && dataset == 0 && precision != 0) {
       const GlLargePlan gp = gl_large_plan(M.glmN, C, D, num_cus());
       DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
       DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
@@ -1276,12 +1464,25 @@ static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n
       gl_large_grad(a, gp, M.low_x(precision, 0), precision, 0);
       HIPC(hipDeviceSynchronize());
     } else if (M.glm && dataset == 0) {
-      const GlmPlan gp = glm_mfma_plan(M.glmN, C, D, num_cus());
-      DevBuf<double> work(gp.work_doubles);
-      GlmArgs ga{M.d_X, M.d_y, M.glmN, D, C, gp.rows_per_chunk, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
-      const int rcg = precision == 0 ? glm_grad_mfma_f64(ga, gp, 0)
-                                     : glm_grad_mfma_f32(ga, gp, M.low_x(precision, 0), precision, kF32Layout, 0);
-      if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+      // reduced precision: the kernel the sampler's autotuner picks for this many points
+      const ModelImpl::GlmTune tn = precision != 0 ? glm_autotune(M, C, precision, 0) : ModelImpl::GlmTune{};
+      if (tn.path == 1) {
+        const GlLargePlan gp = gl_large_plan(M.glmN, C, D, num_cus());
+        DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
+        DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
+        GlLargeArgs a;
+        a.y = M.d_y; a.N = M.glmN; a.D = D; a.C = C; a.s2inv = M.glm_s2inv; a.Q = Q.p; a.ldq = Dp; a.G = G.p;
+        a.V = V.p; a.Qf = qf.p; a.Dq = gp.Dq; a.R = rr.p; a.ldr = gp.ldr; a.Wg = wg.p; a.Wlp = wl.p;
+        gl_large_grad(a, gp, M.low_x(precision, 0), precision, 0);
+      } else {
+        const GlmPlan gp = glm_mfma_plan(M.glmN, C, D, num_cus());
+        DevBuf<double> work(gp.work_doubles);
+        GlmArgs ga{M.d_X, M.d_y, M.glmN, D, C, gp.rows_per_chunk, M.glm_s2inv, Q.p, Dp, G.p, V.p, work.p};
+        const int rcg = precision == 0 ? glm_grad_mfma_f64(ga, gp, 0)
+                        : tn.path == 2 ? glm_grad_mfma_bf16(ga, gp, M.bf16_x(precision, 0), precision, 0)
+                                       : glm_grad_mfma_f32(ga, gp, M.low_x(precision, 0), precision, kF32Layout, 0);
+        if (rcg) throw std::runtime_error("GLM kernel: unsupported D");
+      }
       HIPC(hipDeviceSynchronize());
     } else if (M.kind == MK_HIER_LOGISTIC) {
       // the sampler's batched group kernel (lane = point)
@@ -1318,6 +1519,11 @@ int nuts_model_transform_inits(nuts_model* m, int dataset, const nuts_data* init
     const Dataset& ds = M.ds[dataset];
     const int D = ds.D;
     for (int i = 0; i < D; ++i) out[i] = std::numeric_limits<double>::quiet_NaN();
+    if (M.plugin) {
+      if (!M.pl.transform_inits) throw std::invalid_argument("init: this plugin model has no transform_inits");
+      if (M.pl.transform_inits(M.pl.user, inits, out)) throw std::invalid_argument("init: plugin transform_inits failed");
+      return NUTS_OK;
+    }
     auto take = [&](const std::string& name, int off, int n, int kind) {
       // kind 0 identity, 1 lower=0 (log), 2 (0,1) (logit)
       if (!inits->has(name)) return;
@@ -1396,6 +1602,7 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
     R.init = init;
     R.inv_metric = inv_metric;
     R.diagnostics = out && out->diagnostics;
+    R.energy_error = out && out->energy_error;
     if (R.diagnostics && cfg->flow) throw std::invalid_argument("diagnostic rows are not available with the flow");
     if (init || inv_metric) {
       // per-chain arrays index with a uniform stride: require one dimension
@@ -1417,13 +1624,19 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
       }
       if (out->inv_metric) std::memcpy(out->inv_metric, r.minv.data(), r.minv.size() * sizeof(double));
       if (out->diagnostics) std::memcpy(out->diagnostics, r.diag.data(), r.diag.size() * sizeof(double));
+      if (out->energy_error) std::memcpy(out->energy_error, r.eerr.data(), r.eerr.size() * sizeof(double));
+      if (out->init_rejections)
+        for (int c = 0; c < C; ++c) {
+          out->init_rejections[2 * c] = cfg->flow ? rep.rej_lp[c] : r.st[c].init_rej_lp;
+          out->init_rejections[2 * c + 1] = cfg->flow ? rep.rej_grad[c] : r.st[c].init_rej_grad;
+        }
       (void)moff;
       out->warmup_seconds = r.warm_s;
       out->sampling_seconds = r.samp_s;
       out->leaf_steps = r.leaf_steps;
       out->grad_seconds_per_launch = r.grad_timed ? r.grad_s / r.grad_timed : 0.0;
       out->grad_launches_timed = r.grad_timed;
-      out->grad_chains_per_launch = r.leaf_steps ? (double)r.grad_chains / r.leaf_steps : 0.0;
+      out->grad_chains_per_launch = r.grad_timed ? (double)r.grad_chains / r.grad_timed : 0.0;
       out->tuned_path = r.tuned_path;
       out->tuned_blocks_per_cu = r.tuned_bpc;
       if (cfg->flow) {
@@ -1459,14 +1672,26 @@ int nuts_transition_replay_precision(nuts_model* m, int dataset, int num_chains,
                                      double jitter, int max_depth, const double* inv_metric, const double* q0,
                                      const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,
                                      double* out_q, double* info) {
+  return nuts_transition_replay_driver(m, dataset, num_chains, metric, stepsize, jitter, max_depth, inv_metric, q0, u,
+                                       n_u, z, n_z, precision, NUTS_DRIVER_FUSED, out_q, info);
+}
+
+int nuts_transition_replay_driver(nuts_model* m, int dataset, int num_chains, int metric, double stepsize,
+                                  double jitter, int max_depth, const double* inv_metric, const double* q0,
+                                  const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,
+                                  int driver, double* out_q, double* info) {
   return guarded([&]() {
+    if (driver < NUTS_DRIVER_FUSED || driver > NUTS_DRIVER_LOCKSTEP_BLOCK)
+      throw std::invalid_argument("driver must be 0 (fused), 1 (lockstep, wave per chain) or 2 (lockstep, block)");
     RunSpec R;
     nuts_config_default(&R.cfg);
     R.cfg.precision = precision;
     validate(R.cfg);
     R.cfg.num_warmup = 0; R.cfg.num_samples = 1; R.cfg.adapt_engaged = 0;
     R.cfg.metric = metric; R.cfg.stepsize = stepsize; R.cfg.stepsize_jitter = jitter;
-    R.cfg.max_depth = max_depth; R.cfg.num_chains = num_chains; R.cfg.fused = 1;
+    R.cfg.max_depth = max_depth; R.cfg.num_chains = num_chains;
+    R.cfg.fused = driver == NUTS_DRIVER_FUSED ? 1 : 0;
+    R.block_mode = driver == NUTS_DRIVER_FUSED ? -1 : (driver == NUTS_DRIVER_LOCKSTEP_BLOCK ? 1 : 0);
     R.init = q0; R.inv_metric = inv_metric;
     R.inj_u = u; R.n_u = n_u; R.inj_z = z; R.n_z = n_z;
     R.first_dataset = dataset; R.num_datasets = 1;

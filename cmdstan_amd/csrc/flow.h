@@ -80,6 +80,7 @@ class FlowNet {
   void gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn, float* C,
             int64_t ldc, int M, int N, int K, const int* Mdev, int epi, const float* bias, const float* aux,
             int64_t ldaux, const int* aidx_k = nullptr, const int* aidx_m = nullptr, const int* cidx_n = nullptr);
+  bool fused_ok(int n_rows) const;  // the fused per-layer kernels fit (H, partial workspace)
   void inverse_pass(int rows);     // U_[L] -> U_[0], keeping the training tape
   void row_losses(int rows);       // rowbuf_ = per-row loss after inverse_pass
   void backward_params(int rows);  // parameter gradient of the mean loss into grad_
@@ -103,6 +104,7 @@ class FlowNet {
   double* xrows_ = nullptr;   // R x D (map_rows)
   int64_t* pick_ = nullptr;   // R minibatch rows
   int* db_dev_ = nullptr;     // L
+  double* ld_part_ = nullptr; // L x ceil(Dmax_b / 64) x R: per-tile row sums of s (fused forward)
   int64_t adam_t_ = 0;
 };
 

@@ -136,6 +136,331 @@ __global__ void flow_gemm_reduce(GemmDesc g) {
 // ------------------------------------------------------------ element-wise
 __device__ __forceinline__ int rows_of(const int* n_dev, int n) { return n_dev ? min(n, *n_dev) : n; }
 
+// ------------------------------------------- fused sampling-path layers
+// The sampler's flow gradient (forward T(z) and the pullback) runs each
+// coupling layer as three kernels instead of the generic split-K GEMM chain:
+//   forward   h1 partials (split over the conditioner columns)
+//             h1 = tanh(sum + b1) and h2 = tanh(h1 W2 + b2) per row tile
+//             [s, t] = h2 W3 + b3 with the coupling and log-det epilogue
+//   pullback  coupling cotangents and GP2 partials (GO W3^T, split over the
+//             transformed columns)
+//             GP2 = sum (1 - h2^2), GP1 = (GP2 W2^T)(1 - h1^2)
+//             GC[:, ia] += GP1 W1^T
+// Tiles: 64 rows x 64 columns per block of four waves (2 x 2 of 32 x 32), K in
+// steps of 16 staged through LDS, v_mfma_f32_16x16x4f32.  Every sum has a
+// fixed order (no atomics): results are bit-reproducible.
+#define FF_LD 68       // LDS row stride of the staged operand slabs (floats)
+#define FF_HMAX 256    // widest MLP the fused kernels hold in LDS (64 x H tile)
+
+// acc += A(m, k) B(k, n) over k in [kb, ke) for the block's 64 x 64 tile.
+// A_KC: consecutive k of A are adjacent in memory (a thread loads 4 of one
+// row); B_NC: consecutive n of B are adjacent (a thread loads 4 of one k).
+template <bool A_KC, bool B_NC, class AF, class BF>
+__device__ __forceinline__ void ff_mma(const AF& af, const BF& bf, int kb, int ke, f4 (&acc)[2][2],
+                                       float (*As)[FF_LD], float (*Bs)[FF_LD]) {
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int mm = A_KC ? (t >> 2) : ((t & 15) * 4 + e);
+      const int kk = A_KC ? ((t & 3) * 4 + e) : (t >> 4);
+      As[kk][mm] = (k0 + kk < ke) ? af(mm, k0 + kk) : 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int nn = B_NC ? ((t & 15) * 4 + e) : (t >> 2);
+      const int kk = B_NC ? (t >> 4) : ((t & 3) * 4 + e);
+      Bs[kk][nn] = (k0 + kk < ke) ? bf(k0 + kk, nn) : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int kr = ks * 4 + (l >> 4);
+      const float a0 = As[kr][wm + (l & 15)], a1 = As[kr][wm + 16 + (l & 15)];
+      const float b0 = Bs[kr][wn + (l & 15)], b1 = Bs[kr][wn + 16 + (l & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void ff_zero(f4 (&acc)[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0, 0, 0, 0};
+}
+
+// Calls f(mm, nn, v) for every element of the block's 64 x 64 tile (mm, nn
+// local): register r of MFMA tile (i, j) of lane l holds row
+// wm + 16 i + 4 (l / 16) + r, column wn + 16 j + l % 16.
+template <class F>
+__device__ __forceinline__ void ff_store(const f4 (&acc)[2][2], const F& f) {
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) f(wm + 16 * i + 4 * (l >> 4) + r, wn + 16 * j + (l & 15), acc[i][j][r]);
+}
+
+// forward 1: h1 pre-activations U[:, ia] W1 over the conditioner columns
+// [z kper, (z+1) kper); with one split the epilogue writes h1 = tanh(. + b1)
+__global__ void __launch_bounds__(256) ff_h1(const float* __restrict__ U, int D, const int* __restrict__ ia, int Da,
+                                             const float* __restrict__ W1, const float* __restrict__ b1, int H,
+                                             const int* __restrict__ n_dev, int n, int kper, int splits,
+                                             float* __restrict__ h1, float* __restrict__ part) {
+  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+  const int M = rows_of(n_dev, n);
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  if (m0 >= M) return;
+  const int kb = blockIdx.z * kper, ke = min(Da, kb + kper);
+  f4 acc[2][2];
+  ff_zero(acc);
+  auto af = [&](int mm, int k) { const int m = m0 + mm; return m < M ? U[(int64_t)m * D + ia[k]] : 0.0f; };
+  auto bf = [&](int k, int nn) { const int c = n0 + nn; return c < H ? W1[(int64_t)k * H + c] : 0.0f; };
+  ff_mma<true, true>(af, bf, kb, ke, acc, As, Bs);
+  ff_store(acc, [&](int mm, int nn, float v) {
+    const int m = m0 + mm, c = n0 + nn;
+    if (m >= M || c >= H) return;
+    if (splits == 1) h1[(int64_t)m * H + c] = tanhf(v + b1[c]);
+    else part[((int64_t)blockIdx.z * n + m) * H + c] = v;
+  });
+}
+
+// forward 2: h1 = tanh(sum of the partials + b1) for the block's rows (all H
+// columns, kept in LDS; column tile 0 stores it), h2 = tanh(h1 W2 + b2)
+__global__ void __launch_bounds__(256) ff_h2(const float* __restrict__ part, int splits, const float* __restrict__ b1,
+                                             float* __restrict__ h1, const float* __restrict__ W2,
+                                             const float* __restrict__ b2, float* __restrict__ h2, int H,
+                                             const int* __restrict__ n_dev, int n) {
+  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+  __shared__ float sh[64][FF_HMAX + 1];
+  const int M = rows_of(n_dev, n);
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  if (m0 >= M) return;
+  for (int i = threadIdx.x; i < 64 * H; i += 256) {
+    const int mm = i / H, c = i % H, m = m0 + mm;
+    float v = 0.0f;
+    if (m < M) {
+      if (splits == 1) {
+        v = h1[(int64_t)m * H + c];
+      } else {
+        float s = 0.0f;
+        for (int z = 0; z < splits; ++z) s += part[((int64_t)z * n + m) * H + c];
+        v = tanhf(s + b1[c]);
+        if (blockIdx.x == 0) h1[(int64_t)m * H + c] = v;
+      }
+    }
+    sh[mm][c] = v;
+  }
+  __syncthreads();
+  f4 acc[2][2];
+  ff_zero(acc);
+  auto af = [&](int mm, int k) { return sh[mm][k]; };
+  auto bf = [&](int k, int nn) { const int c = n0 + nn; return c < H ? W2[(int64_t)k * H + c] : 0.0f; };
+  ff_mma<true, true>(af, bf, 0, H, acc, As, Bs);
+  ff_store(acc, [&](int mm, int nn, float v) {
+    const int m = m0 + mm, c = n0 + nn;
+    if (m < M && c < H) h2[(int64_t)m * H + c] = tanhf(v + b2[c]);
+  });
+}
+
+// forward 3: [s_raw, t] = h2 W3 + b3 for the transformed columns
+// [j0, j0 + 64) (s at column j, t at Db + j), then the coupling
+// u_b e^s + t with s = 3 tanh(s_raw / 3), S = s, the row sums of s over the
+// tile (ld_part, FP64) and this block's share of the conditioner columns
+// copied through
+__global__ void __launch_bounds__(256) ff_couple(const float* __restrict__ h2, int H, const float* __restrict__ W3,
+                                                 const float* __restrict__ b3, const int* __restrict__ ia, int Da,
+                                                 const int* __restrict__ ib, int Db, int D,
+                                                 const float* __restrict__ U, float* __restrict__ Un,
+                                                 float* __restrict__ S, int64_t lds, double* __restrict__ ld_part,
+                                                 const int* __restrict__ n_dev, int n) {
+  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+  __shared__ float ss[64][65];
+  const int M = rows_of(n_dev, n);
+  const int m0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  if (m0 >= M) return;
+  const int nt = gridDim.x;
+  // conditioner columns pass through unchanged
+  const int share = (Da + nt - 1) / nt, ja = blockIdx.x * share, jz = min(Da, ja + share);
+  for (int i = threadIdx.x; i < 64 * (jz > ja ? jz - ja : 0); i += 256) {
+    const int mm = i / (jz - ja), j = ja + i % (jz - ja), m = m0 + mm;
+    if (m < M) Un[(int64_t)m * D + ia[j]] = U[(int64_t)m * D + ia[j]];
+  }
+  const int64_t ld3 = 2 * (int64_t)Db;
+  auto af = [&](int mm, int k) { const int m = m0 + mm; return m < M ? h2[(int64_t)m * H + k] : 0.0f; };
+  f4 as[2][2], at[2][2];
+  ff_zero(as);
+  ff_zero(at);
+  auto bs = [&](int k, int nn) { const int j = j0 + nn; return j < Db ? W3[k * ld3 + j] : 0.0f; };
+  auto bt = [&](int k, int nn) { const int j = j0 + nn; return j < Db ? W3[k * ld3 + Db + j] : 0.0f; };
+  ff_mma<true, true>(af, bs, 0, H, as, As, Bs);
+  ff_mma<true, true>(af, bt, 0, H, at, As, Bs);
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mm = wm + 16 * i + 4 * (l >> 4) + r, nn = wn + 16 * jt + (l & 15);
+        const int m = m0 + mm, j = j0 + nn;
+        float sv = 0.0f;
+        if (m < M && j < Db) {
+          sv = FLOW_S * tanhf((as[i][jt][r] + b3[j]) / FLOW_S);
+          const int d = ib[j];
+          Un[(int64_t)m * D + d] = U[(int64_t)m * D + d] * expf(sv) + (at[i][jt][r] + b3[Db + j]);
+          S[(int64_t)m * lds + j] = sv;
+        }
+        ss[mm][nn] = sv;
+      }
+  __syncthreads();
+  if (t < 64 && m0 + t < M) {
+    double acc = 0.0;
+    for (int nn = 0; nn < 64; ++nn) acc += (double)ss[t][nn];
+    ld_part[(int64_t)blockIdx.x * n + m0 + t] = acc;
+  }
+}
+
+// forward end: logdet = sum over layers and tiles of the row sums + sum log
+// scale; X = loc + scale * U_L
+__global__ void ff_finish(const double* __restrict__ ld_part, const int* __restrict__ Db, int L, int tmax,
+                          const float* __restrict__ UL, const double* __restrict__ loc,
+                          const double* __restrict__ scale, int D, const int* __restrict__ n_dev, int n,
+                          double log_scale_sum, double* __restrict__ logdet, double* __restrict__ X) {
+  const int r = blockIdx.x;
+  if (r >= rows_of(n_dev, n)) return;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int l = 0; l < L; ++l) {
+      const int nt = (Db[l] + 63) / 64;
+      for (int tt = 0; tt < nt; ++tt) s += ld_part[((int64_t)l * tmax + tt) * n + r];
+    }
+    logdet[r] = s + log_scale_sum;
+  }
+  for (int j = threadIdx.x; j < D; j += blockDim.x)
+    X[(int64_t)r * D + j] = loc[j] + scale[j] * (double)UL[(int64_t)r * D + j];
+}
+
+// pullback 1 (one layer, transformed columns [j0, j0 + 64)): the coupling
+// cotangents GO = [dU/ds_raw, dU/dt] (dU/ds = gv u e^s - 1, the -1 from the
+// log-det), GC_b <- gv e^s in place, and the partial GP2 = GO W3^T over these
+// columns for every MLP unit
+__global__ void __launch_bounds__(256) fb_couple(float* __restrict__ GC, const float* __restrict__ U,
+                                                 const float* __restrict__ S, int64_t lds,
+                                                 const int* __restrict__ ib, int Db, int D,
+                                                 const float* __restrict__ W3, int H, float* __restrict__ part,
+                                                 const int* __restrict__ n_dev, int n) {
+  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+  __shared__ float go[64][129];
+  const int M = rows_of(n_dev, n);
+  const int m0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  if (m0 >= M) return;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int mm = i >> 6, jj = i & 63, m = m0 + mm, j = j0 + jj;
+    float gs = 0.0f, gt = 0.0f;
+    if (m < M && j < Db) {
+      const int64_t e = (int64_t)m * D + ib[j];
+      const float s = S[(int64_t)m * lds + j];
+      const float es = expf(s);
+      const float gv = GC[e];
+      const float q = s / FLOW_S;
+      gs = (gv * U[e] * es - 1.0f) * (1.0f - q * q);
+      gt = gv;
+      GC[e] = gv * es;
+    }
+    go[mm][jj] = gs;
+    go[mm][64 + jj] = gt;
+  }
+  __syncthreads();
+  const int64_t ld3 = 2 * (int64_t)Db;
+  auto af = [&](int mm, int k) { return go[mm][k]; };
+  for (int c0 = 0; c0 < H; c0 += 64) {
+    f4 acc[2][2];
+    ff_zero(acc);
+    // B(k, c) = W3[c][j], k < 64: s column j0 + k, else t column Db + j0 + k - 64
+    auto bf = [&](int k, int nn) {
+      const int c = c0 + nn;
+      const int j = j0 + (k & 63);
+      if (c >= H || j >= Db) return 0.0f;
+      return W3[(int64_t)c * ld3 + (k < 64 ? j : Db + j)];
+    };
+    ff_mma<true, false>(af, bf, 0, 128, acc, As, Bs);
+    ff_store(acc, [&](int mm, int nn, float v) {
+      const int m = m0 + mm, c = c0 + nn;
+      if (m < M && c < H) part[((int64_t)blockIdx.x * n + m) * H + c] = v;
+    });
+  }
+}
+
+// pullback 2: GP2 = (sum of the partials)(1 - h2^2) for the block's rows (all
+// H in LDS), GP1 = (GP2 W2^T)(1 - h1^2) for the block's unit columns
+__global__ void __launch_bounds__(256) fb_mlp(const float* __restrict__ part, int splits,
+                                              const float* __restrict__ h1, const float* __restrict__ h2,
+                                              const float* __restrict__ W2, int H, float* __restrict__ GP1,
+                                              const int* __restrict__ n_dev, int n) {
+  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+  __shared__ float sg[64][FF_HMAX + 1];
+  const int M = rows_of(n_dev, n);
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  if (m0 >= M) return;
+  for (int i = threadIdx.x; i < 64 * H; i += 256) {
+    const int mm = i / H, c = i % H, m = m0 + mm;
+    float v = 0.0f;
+    if (m < M) {
+      float s = 0.0f;
+      for (int z = 0; z < splits; ++z) s += part[((int64_t)z * n + m) * H + c];
+      const float hv = h2[(int64_t)m * H + c];
+      v = s * (1.0f - hv * hv);
+    }
+    sg[mm][c] = v;
+  }
+  __syncthreads();
+  f4 acc[2][2];
+  ff_zero(acc);
+  auto af = [&](int mm, int k) { return sg[mm][k]; };
+  // B(k, c) = W2[c][k] (h2 = h1 W2: W2[k_in * H + k_out])
+  auto bf = [&](int k, int nn) { const int c = n0 + nn; return c < H ? W2[(int64_t)c * H + k] : 0.0f; };
+  ff_mma<true, false>(af, bf, 0, H, acc, As, Bs);
+  ff_store(acc, [&](int mm, int nn, float v) {
+    const int m = m0 + mm, c = n0 + nn;
+    if (m < M && c < H) {
+      const float hv = h1[(int64_t)m * H + c];
+      GP1[(int64_t)m * H + c] = v * (1.0f - hv * hv);
+    }
+  });
+}
+
+// pullback 3: GC[:, ia[j]] += GP1 W1^T for the conditioner columns
+// [j0, j0 + 64) (each element owned by one block)
+__global__ void __launch_bounds__(256) fb_in(const float* __restrict__ GP1, int H, const float* __restrict__ W1,
+                                             const int* __restrict__ ia, int Da, int D, float* __restrict__ GC,
+                                             const int* __restrict__ n_dev, int n) {
+  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+  const int M = rows_of(n_dev, n);
+  const int m0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  if (m0 >= M) return;
+  f4 acc[2][2];
+  ff_zero(acc);
+  auto af = [&](int mm, int k) { const int m = m0 + mm; return m < M ? GP1[(int64_t)m * H + k] : 0.0f; };
+  // B(k, j) = W1[j][k] (W1 [Da][H])
+  auto bf = [&](int k, int nn) { const int j = j0 + nn; return j < Da ? W1[(int64_t)j * H + k] : 0.0f; };
+  ff_mma<true, false>(af, bf, 0, H, acc, As, Bs);
+  ff_store(acc, [&](int mm, int nn, float v) {
+    const int m = m0 + mm, j = j0 + nn;
+    if (m < M && j < Da) GC[(int64_t)m * D + ia[j]] += v;
+  });
+}
+
 // U0[r][j] = Q[idx[r]][j]
 __global__ void flow_gather_z(const double* __restrict__ Q, int64_t ldq, const int* __restrict__ idx,
                               const int* __restrict__ n_dev, int n, int D, float* __restrict__ U0) {
@@ -453,6 +778,7 @@ FlowNet::FlowNet(const FlowSpec& spec, int max_rows) : spec_(spec), R_(max_rows)
   part_ = dalloc<float>(part_n_);
   std::vector<int> dbs(L);
   for (int l = 0; l < L; ++l) dbs[l] = layers_[l].Db;
+  ld_part_ = dalloc<double>((int64_t)L * ((Dmax_b_ + 63) / 64) * R);
   db_dev_ = dalloc<int>(L);
   HIPF(hipMemcpy(db_dev_, dbs.data(), L * sizeof(int), hipMemcpyHostToDevice));
   std::vector<double> z(D, 0.0), o(D, 1.0);
@@ -469,7 +795,7 @@ FlowNet::~FlowNet() {
   for (void* p : {(void*)params_, (void*)grad_, (void*)adam_m_, (void*)adam_v_, (void*)loc_, (void*)scale_, (void*)U_,
                   (void*)H1_, (void*)H2_, (void*)O_, (void*)S_, (void*)GO_, (void*)GP2_, (void*)GP1_,
                   (void*)GC_, (void*)part_, (void*)rowbuf_, (void*)rowbuf2_, (void*)pick_, (void*)xrows_,
-                  (void*)db_dev_})
+                  (void*)db_dev_, (void*)ld_part_})
     if (p) (void)hipFree(p);
 }
 
@@ -552,12 +878,49 @@ void FlowNet::mlp(int l, const float* U, int ldu, const int* ia, const int* n_de
        nullptr, 0);
 }
 
+// split of the first MLP product over the conditioner columns: about 256
+// blocks, at least 32 columns per split, partials within the workspace
+static int ff_h1_splits(int Da, int H, int n_rows, int64_t part_n, int* kper) {
+  const int tiles = ((H + 63) / 64) * ((n_rows + 63) / 64);
+  int splits = std::max(1, std::min((256 + tiles - 1) / tiles, Da / 32));
+  while (splits > 1 && (int64_t)splits * n_rows * H > part_n) --splits;
+  *kper = ((Da + splits - 1) / splits + 15) / 16 * 16;
+  return (Da + *kper - 1) / *kper;
+}
+
+bool FlowNet::fused_ok(int n_rows) const {
+  const int64_t rh = (int64_t)n_rows * spec_.H;
+  return spec_.H <= FF_HMAX && (int64_t)((Dmax_b_ + 63) / 64) * rh <= part_n_;
+}
+
 void FlowNet::forward(const double* Q, int64_t ldq, const int* idx, const int* n_dev, int n_rows, double* X,
                       double* logdet) {
   if (n_rows > R_) throw std::invalid_argument("flow: too many rows");
-  const int D = spec_.D, L = spec_.L;
+  const int D = spec_.D, L = spec_.L, H = spec_.H;
   const int64_t RD = (int64_t)R_ * D;
   hipLaunchKernelGGL(flow_gather_z, grid1((int64_t)n_rows * D), dim3(256), 0, st_, Q, ldq, idx, n_dev, n_rows, D, U_);
+  if (fused_ok(n_rows)) {
+    const float* P = params_;
+    const int tm = (n_rows + 63) / 64, th = (H + 63) / 64, tmax = (Dmax_b_ + 63) / 64;
+    for (int l = 0; l < L; ++l) {
+      const Layer& ly = layers_[l];
+      float* U = U_ + l * RD;
+      float* h1 = H1_ + (int64_t)l * R_ * H;
+      float* h2 = H2_ + (int64_t)l * R_ * H;
+      int kper = 0;
+      const int splits = ff_h1_splits(ly.Da, H, n_rows, part_n_, &kper);
+      hipLaunchKernelGGL(ff_h1, dim3(th, tm, splits), dim3(256), 0, st_, U, D, ly.ia, ly.Da, P + ly.off[0],
+                         P + ly.off[1], H, n_dev, n_rows, kper, splits, h1, part_);
+      hipLaunchKernelGGL(ff_h2, dim3(th, tm), dim3(256), 0, st_, part_, splits, P + ly.off[1], h1, P + ly.off[2],
+                         P + ly.off[3], h2, H, n_dev, n_rows);
+      hipLaunchKernelGGL(ff_couple, dim3((ly.Db + 63) / 64, tm), dim3(256), 0, st_, h2, H, P + ly.off[4],
+                         P + ly.off[5], ly.ia, ly.Da, ly.ib, ly.Db, D, U, U + RD, S_ + (int64_t)l * R_ * Dmax_b_,
+                         (int64_t)ly.Db, ld_part_ + (int64_t)l * tmax * n_rows, n_dev, n_rows);
+    }
+    hipLaunchKernelGGL(ff_finish, dim3(n_rows), dim3(256), 0, st_, ld_part_, db_dev_, L, tmax, U_ + L * RD, loc_,
+                       scale_, D, n_dev, n_rows, log_scale_sum_, logdet, X);
+    return;
+  }
   for (int l = 0; l < L; ++l) {
     const Layer& ly = layers_[l];
     float* U = U_ + l * RD;
@@ -576,6 +939,25 @@ void FlowNet::pullback(const double* GX, const double* VX, const double* logdet,
   const int D = spec_.D, L = spec_.L, H = spec_.H;
   const int64_t RD = (int64_t)R_ * D;
   hipLaunchKernelGGL(flow_bwd_init, grid1((int64_t)n_rows * D), dim3(256), 0, st_, GX, scale_, D, n_dev, n_rows, GC_);
+  if (fused_ok(n_rows)) {
+    const float* P = params_;
+    const int tm = (n_rows + 63) / 64, th = (H + 63) / 64;
+    for (int l = L - 1; l >= 0; --l) {
+      const Layer& ly = layers_[l];
+      const float* h1 = H1_ + (int64_t)l * R_ * H;
+      const float* h2 = H2_ + (int64_t)l * R_ * H;
+      const int tb = (ly.Db + 63) / 64;
+      hipLaunchKernelGGL(fb_couple, dim3(tb, tm), dim3(256), 0, st_, GC_, U_ + l * RD, S_ + (int64_t)l * R_ * Dmax_b_,
+                         (int64_t)ly.Db, ly.ib, ly.Db, D, P + ly.off[4], H, part_, n_dev, n_rows);
+      hipLaunchKernelGGL(fb_mlp, dim3(th, tm), dim3(256), 0, st_, part_, tb, h1, h2, P + ly.off[2], H, GP1_, n_dev,
+                         n_rows);
+      hipLaunchKernelGGL(fb_in, dim3((ly.Da + 63) / 64, tm), dim3(256), 0, st_, GP1_, H, P + ly.off[0], ly.ia, ly.Da,
+                         D, GC_, n_dev, n_rows);
+    }
+    hipLaunchKernelGGL(flow_finish, grid1((int64_t)n_rows * D), dim3(256), 0, st_, GC_, VX, logdet, idx, n_dev,
+                       n_rows, D, G, ldg, V);
+    return;
+  }
   for (int l = L - 1; l >= 0; --l) {
     const Layer& ly = layers_[l];
     const float* P = params_;

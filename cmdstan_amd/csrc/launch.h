@@ -15,6 +15,7 @@ enum ModelKind : int {
   MK_HIER_NORMAL_NC,
   MK_HIER_LOGISTIC,
   MK_LOGISTIC,
+  MK_PLUGIN,  // caller-supplied model (nuts_model_plugin): lockstep only, gradient by callback
   MK_COUNT
 };
 
@@ -78,5 +79,13 @@ void glm_convert_x(const double* X, void* out, int64_t n, int precision, hipStre
 int glm_grad_mfma_f32(const GlmArgs& a, const GlmPlan& p, const void* Xlow, int precision, int layout,
                       hipStream_t st);
 int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st);
+
+// BF16 matrix-core version with FP32-level accuracy: every FP32 operand as three
+// BF16 terms (X as three terms for precision 1, one exact term when X is stored
+// in BF16, precision 2).  glm_bf16_prepare_x writes the terms of X into `out`
+// (precision 1: 3 x N x ldx, precision 2: N x ldx values, ldx = glm_bf16_ldx(D)).
+int glm_bf16_ldx(int D);
+void glm_bf16_prepare_x(const double* X, int N, int D, int precision, unsigned short* out, hipStream_t st);
+int glm_grad_mfma_bf16(const GlmArgs& a, const GlmPlan& p, const unsigned short* Xs, int precision, hipStream_t st);
 
 }  // namespace nutsgpu

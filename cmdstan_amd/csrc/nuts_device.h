@@ -526,6 +526,8 @@ struct NutsWave {
       if (P.adapt) { s.ss_return = SS_RET_TRANS; s.phase = PH_SS_START; }
       else s.phase = PH_TRANS_BEGIN;
     } else {
+      if (!isfinite(V)) s.init_rej_lp++;
+      else s.init_rej_grad++;
       s.init_tries++;
       if (s.init_tries >= P.max_init_tries) fail(FAIL_INIT);
       else s.phase = PH_INIT_TRY;
@@ -619,6 +621,7 @@ struct NutsWave {
     s.sum_metro = 0;
     s.depth = 0;
     s.divergent = 0;
+    s.max_abs_dH = 0;
     s.phase = PH_DOUBLING_BEGIN;
   }
 
@@ -672,6 +675,7 @@ struct NutsWave {
     }
     if (isnan(h)) h = __builtin_inf();
     if ((h - s.H0) > P.max_deltaH) s.divergent = 1;
+    s.max_abs_dH = fmax(s.max_abs_dH, fabs(h - s.H0));
     const double lw = s.H0 - h;
     if (s.H0 - h > 0) s.sum_metro += 1;
     else s.sum_metro += exp(s.H0 - h);
@@ -899,6 +903,7 @@ struct NutsWave {
         row[4] = s.n_leapfrog;
         row[5] = s.divergent;
         row[6] = s.slot_H[k];
+        if (B.eerr) B.eerr[(int64_t)c * P.out_rows + s.draw_idx] = s.max_abs_dH;
       } converge();
       const VT* sq = Sq(k);
       for (int i = tid; i < nsave; i += TPC) row[7 + i] = sq[i]; converge();

@@ -55,12 +55,14 @@ struct ChainState {
   int divergent, dir, z_is_fwd, init_tries;
   int ss_dir, ss_first, sample_slot, s_prop;
   uint32_t free_mask, w_counter, w_size, w_next;
-  int adapt_flag, welford_n, pad0, pad1;
+  int adapt_flag, welford_n;
+  int init_rej_lp, init_rej_grad;  // initial values rejected: non-finite log density / gradient
   int L_prop[NUTS_MAX_DEPTH];
   double eps, nom_eps, H0, log_w;
   double sum_metro, Vz, Vo, S_logw;
   double leaf_h, ss_H0, sa_counter, sa_sbar;
-  double sa_xbar, sa_mu, pad2, pad3;
+  double sa_xbar, sa_mu;
+  double max_abs_dH, pad3;  // max |H - H0| over the leaves of the current transition
   double L_logw[NUTS_MAX_DEPTH];
   double slot_V[NUTS_MAX_SLOTS];
   double slot_H[NUTS_MAX_SLOTS];
@@ -80,6 +82,7 @@ struct NutsParams {
   int nslots, maxd, NV, fused_budget;
   int save_params;  // -1: all D coordinates in each draw row; else the first save_params
   int diag;         // 1: record the diagnostic rows (q, p, g of each saved draw)
+  int out_rows, pad;  // saved rows per chain (row stride of eerr)
 };
 
 // Device pointers (kernel argument by value).
@@ -112,6 +115,8 @@ struct NutsBuffers {
   // the sampler columns, then q, p, g of the draw
   double* diag;
   const int64_t* diag_off;
+  // per saved row, the transition's max |H - H0| over its leaves: [C][out_rows] or null
+  double* eerr;
 };
 
 // Vector ids inside a chain's pool block.

@@ -13,7 +13,7 @@ import json
 
 import numpy as np
 
-from ._lib import DP, I64P, last_error, lib, nuts_config, nuts_output
+from ._lib import DP, I64P, INTERRUPT_FN, last_error, lib, nuts_config, nuts_output
 
 SAMPLER_COLUMNS = ["lp__", "accept_stat__", "stepsize__", "treedepth__", "n_leapfrog__", "divergent__", "energy__"]
 METRICS = {"unit_e": 0, "diag_e": 1, "dense_e": 2}
@@ -32,13 +32,16 @@ def _dp(a):
 
 
 class Data:
-    """A var_context: named arrays (from a dict or CmdStan JSON text)."""
+    """A var_context: named arrays (from a dict, CmdStan JSON text or R dump text)."""
 
-    def __init__(self, values=None, json_text=None):
+    def __init__(self, values=None, json_text=None, rdump_text=None):
         L = lib()
         self.h = L.nuts_data_create()
         if json_text is not None:
             if L.nuts_data_parse_json(self.h, json_text.encode()) != 0:
+                raise SamplerError(last_error())
+        if rdump_text is not None:
+            if L.nuts_data_parse_rdump(self.h, rdump_text.encode()) != 0:
                 raise SamplerError(last_error())
         for k, v in (values or {}).items():
             a = np.asarray(v)
@@ -51,6 +54,23 @@ class Data:
     def __del__(self):
         if getattr(self, "h", None):
             lib().nuts_data_destroy(self.h)
+
+    def get(self, name):
+        """The named array (row-major), reshaped to its dimensions."""
+        nd = C.c_int()
+        dims = (C.c_int64 * 8)()
+        vals = DP()
+        if lib().nuts_data_get(self.h, name.encode(), C.byref(nd), dims, C.byref(vals)) != 0:
+            raise SamplerError(last_error())
+        shape = tuple(dims[i] for i in range(nd.value))
+        n = int(np.prod(shape)) if shape else 1
+        return np.array([vals[i] for i in range(n)]).reshape(shape)
+
+
+def data_from_file(path):
+    """get_var_context: a .json file is JSON, any other file R dump."""
+    text = open(path).read()
+    return Data(json_text=text) if path.endswith(".json") else Data(rdump_text=text)
 
 
 class Fit:
@@ -80,6 +100,24 @@ class Model:
             raise SamplerError(last_error())
         self.family = family
         self.num_datasets = len(self._data)
+        self._load_names()
+
+    @classmethod
+    def from_plugin(cls, plugin, keep=None):
+        """A model from a nuts_model_plugin structure (a caller-supplied model
+        with its own device gradient); `keep` holds whatever must outlive the
+        model (the plugin's library handle, its user data)."""
+        m = cls.__new__(cls)
+        m._data, m._keep = [], keep
+        m.h = lib().nuts_model_create_plugin(C.byref(plugin))
+        if not m.h:
+            raise SamplerError(last_error())
+        m.family = "plugin"
+        m.num_datasets = 1
+        m._load_names()
+        return m
+
+    def _load_names(self):
         self.dims, self.num_outputs, self.param_names, self.unconstrained_names = [], [], [], []
         buf = C.create_string_buffer(256)
         for m in range(self.num_datasets):
@@ -136,7 +174,7 @@ class Model:
                window=25, max_depth=10, metric="diag_e", stepsize=1.0, stepsize_jitter=0.0, id=1, seed=0,
                init=2.0, inv_metric=None, fused=-1, precision=0, max_saved_params=-1, flow=False, flow_layers=8,
                flow_hidden=128, flow_train_steps=1000, flow_batch=256, flow_lr=1e-3, flow_warmup=300,
-               flow_pilot=100, refresh=0, diagnostics=False):
+               flow_pilot=100, refresh=0, diagnostics=False, energy_error=False, interrupt=None):
         cfg = nuts_config()
         lib().nuts_config_default(C.byref(cfg))
         cfg.num_samples, cfg.num_warmup, cfg.save_warmup, cfg.thin = num_samples, num_warmup, int(save_warmup), thin
@@ -152,6 +190,9 @@ class Model:
         cfg.flow_train_steps, cfg.flow_batch, cfg.flow_lr = flow_train_steps, flow_batch, flow_lr
         cfg.flow_warmup, cfg.flow_pilot = flow_warmup, flow_pilot
         cfg.refresh = refresh
+        if interrupt is not None:
+            # callbacks::interrupt: a Python callable polled between check intervals; True stops the run
+            cfg.interrupt = INTERRUPT_FN(lambda _user: 1 if interrupt() else 0)
         init_arr = None
         if isinstance(init, (int, float)):
             cfg.init_radius = float(init)
@@ -190,6 +231,10 @@ class Model:
         dcols = [7 + 3 * self.dims[m] for m in ds_of_chain]
         diag = np.zeros(sum(rows * c for c in dcols)) if diagnostics else None
         out.diagnostics = _dp(diag)
+        eerr = np.zeros(C_tot * rows) if energy_error else None
+        out.energy_error = _dp(eerr)
+        rej = np.zeros(2 * C_tot, dtype=np.int32)
+        out.init_rejections = rej.ctypes.data_as(C.POINTER(C.c_int))
         rc = lib().nuts_sample(self.h, C.byref(cfg), _dp(init_arr), _dp(im), C.byref(out))
         err = last_error() if rc != 0 else ""
         per_chain, o, mo, metrics = [], 0, 0, []
@@ -211,7 +256,9 @@ class Model:
                 n = rows * dcols[k]
                 diag_rows.append(diag[o:o + n].reshape(rows, dcols[k]))
                 o += n
-        fit = Fit(draws=per_chain, diagnostics=diag_rows, stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
+        fit = Fit(draws=per_chain, diagnostics=diag_rows,
+                  energy_error=None if eerr is None else [eerr[k * rows:(k + 1) * rows] for k in range(C_tot)],
+                  init_rejections=rej.reshape(C_tot, 2), stepsize=step, inv_metric=metrics, n_leapfrog=nlf, grad_evals=nge,
                   status=status, warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds,
                   leaf_steps=out.leaf_steps, grad_seconds_per_launch=out.grad_seconds_per_launch,
                   grad_launches_timed=out.grad_launches_timed,
@@ -225,11 +272,16 @@ class Model:
                   tuned_blocks_per_cu=out.tuned_blocks_per_cu, column_names=names,
                   dataset_of_chain=ds_of_chain, config=cfg, error=err)
         if rc != 0:
-            raise SamplerError(err)
+            e = SamplerError(err)
+            e.fit = fit  # per-chain status and rejection counts of the failed run
+            raise e
         return fit
 
     def transition_replay(self, q0, uniforms, normals, stepsize, metric="diag_e", inv_metric=None, jitter=0.0,
-                          max_depth=10, dataset=0, precision=0):
+                          max_depth=10, dataset=0, precision=0, driver="fused"):
+        """One transition per chain with injected draws.  driver: "fused" (the
+        per-chain kernel), "lockstep" (a wave per chain and the model's batched
+        gradient kernel) or "lockstep_block" (a 256-thread block per chain)."""
         q0 = _f64(np.atleast_2d(q0))
         nch, D = q0.shape
         u = _f64(np.atleast_2d(uniforms))
@@ -243,10 +295,11 @@ class Model:
         im = _f64(np.broadcast_to(base, (nch, msz)))
         out_q = np.zeros((nch, D))
         info = np.zeros((nch, 9))
-        p = {"fp64": 0, "fp32": 1}.get(precision, precision)
-        rc = lib().nuts_transition_replay_precision(self.h, dataset, nch, m, stepsize, jitter, max_depth, _dp(im),
-                                                    _dp(q0), _dp(u), u.shape[1], _dp(z), z.shape[1], p, _dp(out_q),
-                                                    _dp(info))
+        p = {"fp64": 0, "fp32": 1, "bf16": 2}.get(precision, precision)
+        drv = {"fused": 0, "lockstep": 1, "lockstep_block": 2}.get(driver, driver)
+        rc = lib().nuts_transition_replay_driver(self.h, dataset, nch, m, stepsize, jitter, max_depth, _dp(im),
+                                                 _dp(q0), _dp(u), u.shape[1], _dp(z), z.shape[1], p, drv,
+                                                 _dp(out_q), _dp(info))
         if rc != 0:
             raise SamplerError(last_error())
         keys = ["depth", "n_leapfrog", "divergent", "accept_stat", "energy", "lp", "uniforms_used", "normals_used",

@@ -47,6 +47,10 @@ int nuts_data_add(nuts_data* d, const char* name, int ndim, const int64_t* dims,
 /* Parses CmdStan JSON data text (json_data_handler semantics: numbers, nested
  * arrays, "inf"/"-inf"/"NaN" strings). */
 int nuts_data_parse_json(nuts_data* d, const char* json_text);
+/* Parses R dump text, the other format of get_var_context (command_helper.hpp:
+ * 116-133, stan::io::dump): name <- value with numbers, c(...), a:b and
+ * structure(..., .Dim = c(...)); arrays are stored row-major like JSON ones. */
+int nuts_data_parse_rdump(nuts_data* d, const char* rdump_text);
 int nuts_data_get(const nuts_data* d, const char* name, int* ndim, int64_t* dims /* >= 8 */, const double** values);
 
 /* --------------------------------------------------------------- model */
@@ -63,6 +67,44 @@ int nuts_model_dims(const nuts_model* m, int dataset, int* num_unconstrained, in
 int nuts_model_param_name(const nuts_model* m, int dataset, int i, char* buf, int buflen);
 /* Name of unconstrained coordinate i (log_prob output "g_<name>"). */
 int nuts_model_unconstrained_name(const nuts_model* m, int dataset, int i, char* buf, int buflen);
+
+/* ------------------------------------------------------- model plugins */
+/* A model outside the registered families, supplied by the caller in the role
+ * of stanc's generated model_base (created by new_model, command.hpp:71-72,
+ * 170-171; its log_prob, write_array, transform_inits and *_param_names are
+ * what command_helper.hpp:253-404 and the services call): host callbacks for
+ * the names and transforms, and a log-density gradient the caller builds for
+ * gfx950 and enqueues on the sampler's stream.  The sampler drives plugin
+ * models with its lockstep kernels (the gradient is a separate launch per
+ * step of the state machine, batched over every chain that needs one). */
+typedef struct nuts_model_plugin {
+  int dim;                                /* unconstrained dimension D >= 1 */
+  int num_outputs;                        /* constrained outputs of a draw */
+  const char* const* param_names;         /* [num_outputs], Stan CSV names ("theta", "beta.1") */
+  const char* const* unconstrained_names; /* [dim] (log_prob columns g_<name>) */
+  void* user;                             /* passed back to every callback */
+  /* V = -log p and G = -grad log p, log p in Stan's propto convention plus the
+   * log-Jacobian of the constraining transforms when jacobian = 1, for rows
+   * r < n, n = *d_n when d_n is not NULL (a device count, <= n_rows), else
+   * n_rows.  Point r is row idx[r] of Q (row r when idx is NULL), stride ldq;
+   * its results go to V[idx[r]] and row idx[r] of G (stride ldq).  Q, G, V,
+   * idx and d_n are device pointers.  Enqueue the work on `stream` (a
+   * hipStream_t) without synchronising or allocating; return 0, or nonzero to
+   * report an error.  Required. */
+  int (*log_prob_grad)(void* user, const double* Q, int64_t ldq, const int* idx, const int* d_n, int n_rows,
+                       int jacobian, double* G, double* V, void* stream);
+  /* write_array: constrained outputs out [n][num_outputs] of the unconstrained
+   * host rows q [n][dim].  NULL: the outputs are the unconstrained values. */
+  int (*write_array)(void* user, const double* q, int n, double* out);
+  /* transform_inits: the named constrained values in `inits` to the
+   * unconstrained out[dim], NaN for parameters absent from `inits`.  NULL:
+   * init files cannot be used with this model. */
+  int (*transform_inits)(void* user, const nuts_data* inits, double* out);
+} nuts_model_plugin;
+
+/* Creates a model from a plugin (one dataset; the plugin holds its own data).
+ * The names are copied; the callbacks and `user` must outlive the model. */
+nuts_model* nuts_model_create_plugin(const nuts_model_plugin* plugin);
 
 /* ------------------------------------------------------------ log_prob */
 /* lp and its gradient at n_points unconstrained points of one dataset, computed
@@ -128,6 +170,12 @@ typedef struct nuts_config {
   int flow_pilot;         /* 100 pilot draws per chain for each of the two samplers */
   int refresh;            /* 0 (default): silent; k > 0: progress line on stderr each time the
                              slowest chain passes a multiple of k iterations (CmdStan's refresh=) */
+  /* callbacks::interrupt (command.hpp:186): when not NULL, called on the host
+   * between check intervals of the sampler (every 16 lockstep steps, or every
+   * fused launch); a nonzero return stops the run with NUTS_NOT_OK and the
+   * message "sampling interrupted". */
+  int (*interrupt)(void* user);
+  void* interrupt_user;
 } nuts_config;
 
 void nuts_config_default(nuts_config* cfg);
@@ -159,12 +207,14 @@ typedef struct nuts_output {
   double sampling_seconds; /* ... and of the sampling phase */
   int64_t leaf_steps;      /* lockstep gradient launches (0 in fused mode) */
   double grad_seconds_per_launch; /* lockstep: mean duration of the batched gradient's
-                                     contraction kernel over every launch (warmup and
-                                     sampling), from HIP events on the sampler's
-                                     stream (0 in fused mode) */
+                                     contraction kernel (the whole gradient off the GLM
+                                     path) over the timed launches, from HIP events on
+                                     the sampler's stream (0 in fused mode); with graph
+                                     replay one bundle of 16 launches in 8 is launched
+                                     eagerly and timed, spread over warmup and sampling */
   int64_t grad_launches_timed;
-  double grad_chains_per_launch;  /* lockstep GLM path: mean number of active chains a
-                                     gradient launch evaluated (0 elsewhere) */
+  double grad_chains_per_launch;  /* lockstep: mean number of active chains over the same
+                                     timed launches (0 in fused mode) */
   /* flow = 1 only */
   int flow_used;                  /* 1: the draws come from NUTS on the pulled-back density */
   double flow_train_seconds;      /* GPU training time */
@@ -185,6 +235,15 @@ typedef struct nuts_output {
    * the unconstrained draw q, its momentum p and the potential gradient
    * g = -grad log p (columns named <name>, p_<name>, g_<name>).  Not with flow. */
   double* diagnostics;
+  /* per-transition energy error (no counterpart in CmdStan): when not NULL,
+   * [chains][rows], for each saved row the largest |H - H0| over the leaves of
+   * the trajectory that produced it (H0: the Hamiltonian at the transition's
+   * start; +inf for a divergent leaf).  With the flow: the final sampling phase. */
+  double* energy_error;
+  /* services::util::initialize: when not NULL, [chains][2], the initial values
+   * rejected because the log density [0] or its gradient [1] was not finite
+   * (the "Rejecting initial value" messages of command_test.cpp:151-250). */
+  int* init_rejections;
 } nuts_output;
 
 /* model_base::transform_inits (the `init=` file path of command_helper.hpp:144-245):
@@ -217,6 +276,20 @@ int nuts_transition_replay_precision(nuts_model* m, int dataset, int num_chains,
                                      double jitter, int max_depth, const double* inv_metric, const double* q0,
                                      const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,
                                      double* out_q, double* info);
+
+/* The same with the driver of the state machine chosen: driver 0 the fused
+ * per-chain kernel (gradient inside the state machine), 1 the lockstep kernels
+ * with one wavefront per chain and the model's batched gradient kernel (the
+ * FP64 matrix-core GEMM for logistic regression, the group kernel for the
+ * hierarchical logistic model), 2 the lockstep kernels with one 256-thread
+ * block per chain.  No counterpart in the reference. */
+#define NUTS_DRIVER_FUSED 0
+#define NUTS_DRIVER_LOCKSTEP_WAVE 1
+#define NUTS_DRIVER_LOCKSTEP_BLOCK 2
+int nuts_transition_replay_driver(nuts_model* m, int dataset, int num_chains, int metric, double stepsize,
+                                  double jitter, int max_depth, const double* inv_metric, const double* q0,
+                                  const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,
+                                  int driver, double* out_q, double* info);
 
 /* ------------------------------------------------------------ summary */
 /* stansummary statistics (src/cmdstan/stansummary_helper.hpp:397-428 get_stats)

@@ -408,7 +408,53 @@ struct Gauss : Model {
   void write_array(const double* q, double* out) const override { for (int i = 0; i < D; ++i) out[i] = q[i]; }
 };
 
+// ------------------------------------------------------------------ funnel
+// Neal's funnel (Stan User's Guide, "Reparameterization"): v ~ normal(0, 3),
+// x_k ~ normal(0, exp(v / 2)), k = 1..K; no constraints.  Outputs v, the
+// derived scale sigma = exp(v / 2), then x.  The model of the plugin example.
+struct Funnel : Model {
+  int K;
+  explicit Funnel(const Data& d) {
+    K = d.integer("K");
+    if (K < 1) throw std::invalid_argument("funnel: K must be positive");
+  }
+  std::string name() const override { return "funnel"; }
+  int dim() const override { return K + 1; }
+  int num_out() const override { return K + 2; }
+  std::vector<std::string> out_names() const override {
+    std::vector<std::string> r = {"v", "sigma"};
+    for (int k = 0; k < K; ++k) r.push_back("x." + std::to_string(k + 1));
+    return r;
+  }
+  template <class T> T lp(const T* q, bool) const {
+    using std::exp;
+    const T v = q[0];
+    T r = -v * v / 18.0;
+    const T iv = exp(-v);
+    for (int k = 0; k < K; ++k) r += -0.5 * q[1 + k] * q[1 + k] * iv - 0.5 * v;
+    return r;
+  }
+  double log_prob(const double* q, bool jac) const override { return lp(q, jac); }
+  cplx log_prob_c(const cplx* q, bool jac) const override { return lp(q, jac); }
+  double log_prob_grad(const double* q, double* g, bool jac) const override {
+    const double v = q[0], iv = std::exp(-v);
+    double gv = -v / 9.0;
+    for (int k = 0; k < K; ++k) {
+      gv += 0.5 * q[1 + k] * q[1 + k] * iv - 0.5;
+      g[1 + k] = -q[1 + k] * iv;
+    }
+    g[0] = gv;
+    return lp(q, jac);
+  }
+  void write_array(const double* q, double* out) const override {
+    out[0] = q[0];
+    out[1] = std::exp(0.5 * q[0]);
+    for (int k = 0; k < K; ++k) out[2 + k] = q[1 + k];
+  }
+};
+
 inline std::unique_ptr<Model> make_model(const std::string& name, const Data& d) {
+  if (name == "funnel") return std::unique_ptr<Model>(new Funnel(d));
   if (name == "bernoulli") return std::unique_ptr<Model>(new Bernoulli(d));
   if (name == "logistic_regression") return std::unique_ptr<Model>(new LogisticRegression(d));
   if (name == "eight_schools_cp") return std::unique_ptr<Model>(new EightSchoolsCP(d));

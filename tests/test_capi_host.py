@@ -141,3 +141,49 @@ def test_precision_fp32_config_validates():
     assert ca.lib().nuts_config_validate(C.byref(_cfg(precision=1))) == 0
     assert ca.lib().nuts_config_validate(C.byref(_cfg(precision=3))) == 1
     assert "precision" in ca.last_error()
+
+
+def test_example_plugin_fills_the_plugin_table():
+    """The example plugin library loads without a GPU and fills a
+    nuts_model_plugin table (dimension, names, callbacks)."""
+    from cmdstan_amd._lib import nuts_model_plugin
+    path = os.path.join(os.path.dirname(ca.__file__), "plugins", "libfunnel_plugin.so")
+    L = C.CDLL(path)
+    L.funnel_plugin_create.restype = C.c_void_p
+    L.funnel_plugin_create.argtypes = [C.c_int, C.POINTER(nuts_model_plugin)]
+    L.funnel_plugin_destroy.argtypes = [C.c_void_p]
+    p = nuts_model_plugin()
+    h = L.funnel_plugin_create(3, C.byref(p))
+    assert h and p.dim == 4 and p.num_outputs == 5
+    assert [p.param_names[i].decode() for i in range(5)] == ["v", "sigma", "x.1", "x.2", "x.3"]
+    assert [p.unconstrained_names[i].decode() for i in range(4)] == ["v", "x.1", "x.2", "x.3"]
+    assert bool(p.log_prob_grad) and bool(p.write_array) and bool(p.transform_inits)
+    q = np.array([[0.4, 1.0, -2.0, 0.5]])
+    out = np.zeros((1, 5))
+    assert p.write_array(p.user, q.ctypes.data_as(C.POINTER(C.c_double)), 1,
+                         out.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    assert np.allclose(out, [[0.4, np.exp(0.2), 1.0, -2.0, 0.5]])
+    L.funnel_plugin_destroy(h)
+
+
+def test_rdump_reader_matches_json_twins(golden):
+    """The reference's R dump fixtures read to the same arrays as their JSON
+    twins (column-major R arrays become row-major)."""
+    from cmdstan_amd.sampler import data_from_file
+    a = data_from_file(f"{golden}/bern_constrained_params.R")
+    b = data_from_file(f"{golden}/bern_constrained_params.json")
+    for k in ["theta", "mu_vector", "mu_row_vector", "mu_array_1d", "mu_matrix", "mu_matrix_array"]:
+        assert np.array_equal(a.get(k), b.get(k)), k
+    a = data_from_file(f"{golden}/bern_unconstrained_params.R")
+    b = data_from_file(f"{golden}/bern_unconstrained_params.json")
+    assert np.array_equal(a.get("params_r"), b.get("params_r"))
+    for m in ("proper", "test_model"):
+        a = data_from_file(f"{golden}/{m}.dense_e_metric.dat.R")
+        b = data_from_file(f"{golden}/{m}.dense_e_metric.json")
+        assert np.array_equal(a.get("inv_metric"), b.get("inv_metric"))
+    assert data_from_file(f"{golden}/test_model.diag_e_metric.dat.R").get("inv_metric").tolist() == [1.5, 1.8]
+    s = ca.Data(rdump_text="n <- 3L\nseq <- 2:5\nz <- integer(0)\nm <- structure(1:6, .Dim = c(2L, 3L))\n")
+    assert s.get("seq").tolist() == [2, 3, 4, 5] and s.get("z").shape == (0,)
+    assert s.get("m").tolist() == [[1, 3, 5], [2, 4, 6]] and s.get("n") == 3
+    with pytest.raises(ca.SamplerError):
+        ca.Data(rdump_text="m <- structure(c(1, 2, 3), .Dim = c(2, 2))")

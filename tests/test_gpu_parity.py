@@ -259,11 +259,15 @@ def _bf16_round(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+@pytest.mark.parametrize("path", [0, 1, 2])
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
-def test_glm_gradient_reduced_precision(precision):
+def test_glm_gradient_reduced_precision(precision, path, monkeypatch):
     """FP32 contraction (X in FP32, or stored BF16) with FP64 sums: error of lp
     and gradient against FP64 on the same stored data, relative to the sum of
-    |terms|; tolerance 1e-5 (FP32 products, ~2^-24 per term)."""
+    |terms|; tolerance 1e-5 (FP32 products, ~2^-24 per term).  Every kernel
+    path: 0 single-pass FP32 matrix cores, 1 two-pass, 2 single-pass BF16
+    matrix cores with three-term operands."""
+    monkeypatch.setenv("CMDSTAN_AMD_GLM_PATH", str(path))
     N, D, C = 20000, 100, 130
     rng = np.random.default_rng(7)
     X = rng.standard_normal((N, D))
@@ -281,7 +285,7 @@ def test_glm_gradient_reduced_precision(precision):
     cond_lp = np.abs(y[:, None] * E_).sum(0) + np.logaddexp(0, E_).sum(0)
     err_g = np.max(np.abs(g - g_ref) / cond_g)
     err_lp = np.max(np.abs(lp - lp_ref) / cond_lp)
-    print(f"{precision}: max rel gradient error {err_g:.2e}, lp error {err_lp:.2e} (vs FP64 on stored X)")
+    print(f"{precision} path {path}: max rel gradient error {err_g:.2e}, lp error {err_lp:.2e} (vs FP64 on stored X)")
     assert err_g < 1e-5 and err_lp < 1e-5
     if precision == "bf16":
         # the storage itself perturbs the data by up to 2^-9 relative
@@ -290,8 +294,9 @@ def test_glm_gradient_reduced_precision(precision):
               np.max(np.abs(lp - lp64) / cond_lp))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
-def test_reduced_precision_posterior_matches_fp64(precision):
+@pytest.mark.parametrize("precision,path", [("fp32", 0), ("fp32", 2), ("bf16", 0), ("bf16", 2)])
+def test_reduced_precision_posterior_matches_fp64(precision, path, monkeypatch):
+    monkeypatch.setenv("CMDSTAN_AMD_GLM_PATH", str(path))
     data = logistic(seed=9, N=4000, D=8)
     gm = ca.Model("logistic_regression", data)
     a = gm.sample(num_chains=64, seed=6, precision=0)
@@ -369,15 +374,22 @@ def test_block_per_chain_state_machine_matches_wave_mode(metric, monkeypatch):
         assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"])
 
 
-@pytest.mark.parametrize("family", ["gauss", "hier_logistic_cp"])
+@pytest.mark.parametrize("family", ["gauss", "hier_logistic_cp", "logistic_regression", "logistic_fp32"])
 def test_graph_replay_matches_eager_launches(family, monkeypatch):
     """Lockstep bundles replayed from a captured HIP graph (many replays, with
-    the host's checks and synchronisations between them) give bit-identical
-    draws to the same kernels launched eagerly."""
+    the host's checks, synchronisations and eager timed bundles between them)
+    give bit-identical draws to the same kernels launched eagerly; the GLM
+    families run through the batched matrix-core gradient."""
     from tests.model_cases import gauss, hier_logistic
-    data = gauss(seed=5, D=12) if family == "gauss" else hier_logistic(seed=3, N=400, K=3, G=4)
+    data = {"gauss": lambda: gauss(seed=5, D=12), "hier_logistic_cp": lambda: hier_logistic(seed=3, N=400, K=3, G=4),
+            "logistic_regression": lambda: logistic(seed=4, N=3000, D=20),
+            "logistic_fp32": lambda: logistic(seed=4, N=3000, D=20)}[family]()
+    prec = "fp32" if family == "logistic_fp32" else "fp64"
+    family = "logistic_regression" if family == "logistic_fp32" else family
     m = ca.Model(family, data)
     kw = dict(num_chains=24, num_warmup=150, num_samples=100, seed=9, fused=0)
+    if family == "logistic_regression":
+        kw["precision"] = prec
     monkeypatch.setenv("CMDSTAN_AMD_GRAPH", "0")
     a = m.sample(**kw)
     monkeypatch.setenv("CMDSTAN_AMD_GRAPH", "1")
@@ -385,6 +397,26 @@ def test_graph_replay_matches_eager_launches(family, monkeypatch):
     assert np.all(a.status == 0) and np.all(b.status == 0)
     for x, y in zip(a.draws, b.draws):
         assert np.array_equal(x, y)
+    if family == "logistic_regression":
+        assert b.grad_launches_timed > 0 and b.grad_seconds_per_launch > 0
+
+
+def test_interrupt_callback_stops_the_run():
+    """callbacks::interrupt: polled between check intervals; a True return
+    stops the run with an error, False lets it finish."""
+    from tests.model_cases import gauss
+    m = ca.Model("gauss", gauss(seed=1, D=4))
+    calls = []
+
+    def stop():
+        calls.append(1)
+        return len(calls) >= 3
+
+    with pytest.raises(ca.SamplerError, match="interrupted"):
+        m.sample(num_chains=8, num_warmup=2000, num_samples=2000, fused=0, interrupt=stop)
+    assert len(calls) == 3
+    fit = m.sample(num_chains=8, num_warmup=100, num_samples=100, fused=0, interrupt=lambda: False)
+    assert np.all(fit.status == 0)
 
 
 def test_graph_replay_matches_eager_with_flow(monkeypatch):

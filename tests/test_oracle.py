@@ -72,10 +72,11 @@ def _models(rng):
         ("hier_logistic_cp", dict(N=hl_N, K=K, G=G, X=rng.standard_normal((hl_N, K)),
                                   group=rng.integers(1, G + 1, hl_N), y=(rng.random(hl_N) < 0.5).astype(float))),
         ("gauss", dict(D=3, mu=[1.0, -2.0, 0.5], P=np.array([[2.0, 0.3, 0], [0.3, 1.0, 0.2], [0, 0.2, 0.5]]))),
+        ("funnel", dict(K=4)),
     ]
 
 
-@pytest.mark.parametrize("idx", range(6))
+@pytest.mark.parametrize("idx", range(7))
 def test_gradient_complex_step(idx):
     rng = np.random.default_rng(11 + idx)
     name, data = _models(rng)[idx]
@@ -92,6 +93,18 @@ def test_gradient_complex_step(idx):
             h = 1e-6
             fd = np.array([(m.log_prob(q + h * e, jac) - m.log_prob(q - h * e, jac)) / (2 * h) for e in np.eye(m.dim)])
             assert np.max(np.abs(fd - g)) <= 1e-5 * scale
+
+
+def test_funnel_closed_form():
+    """Neal's funnel: log p = -v^2/18 - sum_k (x_k^2 e^{-v} + v) / 2; the
+    derived output sigma = e^{v/2}."""
+    m = O.Model("funnel", dict(K=2))
+    q = np.array([0.7, -1.1, 0.4])
+    lp, g = m.log_prob_grad(q)
+    v, x = q[0], q[1:]
+    assert lp == pytest.approx(-v * v / 18 - 0.5 * np.sum(x * x) * np.exp(-v) - v, rel=1e-15)
+    assert g == pytest.approx([-v / 9 + 0.5 * np.sum(x * x) * np.exp(-v) - 1.0, *(-x * np.exp(-v))], rel=1e-14)
+    assert m.write_array(q) == pytest.approx([v, np.exp(v / 2), *x], rel=1e-15)
 
 
 def test_logistic_gradient_closed_form():

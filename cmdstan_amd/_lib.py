@@ -76,6 +76,8 @@ SIGNATURES = {
     "nuts_model_unconstrained_name": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "nuts_log_prob_grad": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_int, C.c_int, DP, DP]),
     "nuts_log_prob_grad_precision": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_int, C.c_int, C.c_int, DP, DP]),
+    "nuts_log_prob_grad_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]),
     "nuts_model_transform_inits": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, DP]),
     "nuts_config_default": (None, [C.POINTER(nuts_config)]),
     "nuts_config_validate": (C.c_int, [C.POINTER(nuts_config)]),

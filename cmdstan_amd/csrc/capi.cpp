@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -1707,6 +1708,117 @@ int nuts_transition_replay_driver(nuts_model* m, int dataset, int num_chains, in
       inf[0] = row[3]; inf[1] = row[4]; inf[2] = row[5]; inf[3] = row[1]; inf[4] = row[6]; inf[5] = row[0];
       inf[6] = (double)s.uctr; inf[7] = (double)s.nctr; inf[8] = row[2];
     }
+    return NUTS_OK;
+  });
+}
+
+// -------------------------------------------- allocation-free log_prob_grad
+// nuts_log_prob_grad_noalloc keeps its scratch (the potentials, the per-point
+// dimension / offset tables and the gradient kernels' partial sums) in this
+// file-scope device array: one copy per GPU, held until the process ends.
+// Host code never passes the array's name to a kernel: it passes the per-device
+// address that hipGetSymbolAddress returns.
+#define NUTS_NOALLOC_SCRATCH_BYTES (48ull << 20)
+#define NUTS_NOALLOC_MAX_POINTS 4096
+static __device__ __align__(256) unsigned char scratch[NUTS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;  // device -> address of its copy of `scratch`
+
+// address of the current device's copy of `scratch` (looked up once per device)
+unsigned char* scratch_for_current_device() {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    throw HipError(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  return static_cast<unsigned char*>(p);
+}
+
+// per-point tables of the per-chain model kernels: dimension, row offset of
+// the point (rows of D doubles), dataset block offset
+__global__ void noalloc_tables(int* Dv, int64_t* off, int64_t* dso, int n, int D, int64_t ds_off) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Dv[c] = D;
+  off[c] = (int64_t)c * D;
+  dso[c] = ds_off;
+}
+
+// V = -lp and G = -grad lp (in place in grad) to lp and grad lp
+__global__ void noalloc_finish(const double* V, double* lp, double* grad, int n, int D) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) lp[t] = -V[t];
+  if (t < (int64_t)n * D) grad[t] = -grad[t];
+}
+
+}  // namespace
+
+extern "C" int nuts_log_prob_grad_noalloc(nuts_model* m, int dataset, const double* d_q, int n_points, int jacobian,
+                                          double* d_lp, double* d_grad, void* stream) {
+  return guarded([&]() {
+    ModelImpl& M = m->impl;
+    if (dataset < 0 || dataset >= (int)M.ds.size()) throw std::out_of_range("dataset index");
+    if (n_points < 0 || n_points > NUTS_NOALLOC_MAX_POINTS)
+      throw std::invalid_argument("nuts_log_prob_grad_noalloc: n_points must be in [0, " +
+                                  std::to_string(NUTS_NOALLOC_MAX_POINTS) + "]");
+    if (n_points == 0) return NUTS_OK;
+    if (!d_q || !d_lp || !d_grad) throw std::invalid_argument("nuts_log_prob_grad_noalloc: null device buffer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int D = M.ds[dataset].D, n = n_points;
+    unsigned char* base = scratch_for_current_device();
+    // scratch regions, 256-byte aligned: V [n], D [n], offsets [n], dataset offsets [n], kernel partials
+    size_t at = 0;
+    auto region = [&](size_t bytes) {
+      unsigned char* p = base + at;
+      at += (bytes + 255) & ~size_t(255);
+      return p;
+    };
+    double* V = reinterpret_cast<double*>(region(sizeof(double) * n));
+    int* Dv = reinterpret_cast<int*>(region(sizeof(int) * n));
+    int64_t* off = reinterpret_cast<int64_t*>(region(sizeof(int64_t) * n));
+    int64_t* dso = reinterpret_cast<int64_t*>(region(sizeof(int64_t) * n));
+    double* work = reinterpret_cast<double*>(base + at);
+    const size_t work_cap = (NUTS_NOALLOC_SCRATCH_BYTES - at) / sizeof(double);
+    auto need = [&](size_t doubles) {
+      if (doubles > work_cap)
+        throw std::invalid_argument("nuts_log_prob_grad_noalloc: " + std::to_string(n) + " points need " +
+                                    std::to_string(doubles * 8) + " bytes of scratch, more than the " +
+                                    std::to_string(work_cap * 8) + " available; use fewer points");
+    };
+    if (M.plugin) {
+      if (M.pl.log_prob_grad(M.pl.user, d_q, D, nullptr, nullptr, n, jacobian ? 1 : 0, d_grad, V, st))
+        throw std::runtime_error("plugin log_prob_grad failed");
+    } else if (M.glm && dataset == 0) {
+      // the plan of nuts_log_prob_grad, so both give the same sums
+      const GlmPlan gp = glm_mfma_plan(M.glmN, n, D, num_cus());
+      need(gp.work_doubles);
+      GlmArgs ga{M.d_X, M.d_y, M.glmN, D, n, gp.rows_per_chunk, M.glm_s2inv, d_q, D, d_grad, V, work};
+      if (glm_grad_mfma_f64(ga, gp, st)) throw std::runtime_error("GLM kernel: unsupported D");
+    } else if (M.kind == MK_HIER_LOGISTIC) {
+      const Dataset& hd = M.ds[dataset];
+      need(hl_work_doubles(hd.groups, hd.group_size, n));
+      HlArgs a{};
+      a.block = M.d_data.p + M.ds_off[dataset];
+      a.Q = d_q; a.ldq = D; a.idx = nullptr; a.idx_in = false; a.idx_out = false; a.n_dev = nullptr; a.n_rows = n;
+      a.Gout = d_grad; a.ldg = D; a.Vout = V; a.work = work; a.jac = jacobian ? 1.0 : 0.0;
+      hl_grad_launch(a, hd.groups, st);
+    } else {
+      hipLaunchKernelGGL(noalloc_tables, dim3((n + 255) / 256), dim3(256), 0, st, Dv, off, dso, n, D,
+                         M.ds_off[dataset]);
+      ModelDataDev md{M.d_data.p, dso, jacobian ? 1.0 : 0.0};
+      launch_model_grad(M.kind, n, Dv, off, d_q, d_grad, V, nullptr, md, st);
+    }
+    const int64_t tot = std::max<int64_t>((int64_t)n * D, n);
+    hipLaunchKernelGGL(noalloc_finish, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, V, d_lp, d_grad, n, D);
+    HIPC(hipGetLastError());
     return NUTS_OK;
   });
 }

@@ -118,6 +118,19 @@ int nuts_log_prob_grad(nuts_model* m, int dataset, const double* q, int n_points
 int nuts_log_prob_grad_precision(nuts_model* m, int dataset, const double* q, int n_points, int jacobian,
                                  int precision, double* lp, double* grad);
 
+/* nuts_log_prob_grad over caller-owned DEVICE buffers, allocating nothing:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  d_q [n_points][D], d_lp [n_points], d_grad [n_points][D]
+ * are device pointers; the work is enqueued on `stream` (a hipStream_t, NULL
+ * for the null stream) and the call returns without waiting for it.  Scratch
+ * is a fixed 48 MiB device array per GPU, held until the process ends;
+ * n_points is at most 4096 (fewer when a model's partial sums need more
+ * scratch: the call then fails with a message).  Calls that overlap on one
+ * GPU share that scratch and race: serialise them on one stream. */
+int nuts_log_prob_grad_noalloc(nuts_model* m, int dataset, const double* d_q, int n_points, int jacobian,
+                               double* d_lp, double* d_grad, void* stream);
+
 /* -------------------------------------------------------------- sample */
 /* Argument tree of `sample ... algorithm=hmc engine=nuts` with CmdStan defaults
  * (arg_sample.hpp:18-29, arg_adapt.hpp:19-37, arg_hmc.hpp:19-27,

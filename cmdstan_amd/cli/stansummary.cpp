@@ -10,6 +10,7 @@
 #include <iomanip>
 #include <iostream>
 #include <map>
+#include <set>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -133,25 +134,42 @@ int main(int argc, char** argv) {
       std::cout << "Option --autocorr: " << autocorr << " not a valid chain id." << std::endl;
       return 1;
     }
-    // percentiles: numbers in [0, 100], strictly increasing (stansummary.cpp:98-110)
+    // percentiles: the spec is trimmed and split on commas and spaces (runs of
+    // separators count once); each value is read as a number prefix, must be
+    // finite, within [0, 100] and not below the integer part of the one
+    // before it; any violation is one message (stansummary.cpp:98-110,
+    // stansummary_helper.hpp:300-320)
     std::vector<std::string> pcts;
-    double prev = -1.0;
-    for (auto& p : split(pct_spec, ',')) {
-      std::string t = trim(p);
-      t.erase(std::remove(t.begin(), t.end(), '"'), t.end());
-      if (t.empty()) continue;
-      char* end = nullptr;
-      const double x = std::strtod(t.c_str(), &end);
-      if (*end != '\0' || !(x >= 0 && x <= 100)) {
-        std::cout << "Option --percentiles " << pct_spec << ": values must be in range [0, 100]" << std::endl;
+    {
+      const std::string spec = trim(pct_spec);
+      std::string tok;
+      auto flush = [&]() {
+        if (!tok.empty()) pcts.push_back(tok);
+        tok.clear();
+      };
+      for (char ch : spec) {
+        if (ch == ',' || ch == ' ') flush();
+        else tok += ch;
+      }
+      flush();
+      int cur = 0;
+      for (auto& t : pcts) {
+        char* end = nullptr;
+        const double x = std::strtod(t.c_str(), &end);
+        if (end == t.c_str() || !std::isfinite(x) || x < 0.0 || x > 100.0 || x < cur) {
+          std::cout << "Option --percentiles " << spec
+                    << ": values must be in range (0, 100), inclusive, and strictly increasing." << std::endl;
+          return 1;
+        }
+        cur = (int)x;
+      }
+    }
+    if (have_csv) {  // the summary csv is never written over an existing file
+      std::ifstream probe(csv_filename);
+      if (probe.good()) {
+        std::cout << "--csv_filename: Path already exists: " << csv_filename << std::endl;
         return 1;
       }
-      if (!(x > prev)) {
-        std::cout << "Option --percentiles " << pct_spec << ": values must be in ascending order" << std::endl;
-        return 1;
-      }
-      prev = x;
-      pcts.push_back(t);
     }
     for (auto& f : files) {
       if (!std::ifstream(f).good()) {
@@ -168,31 +186,26 @@ int main(int argc, char** argv) {
       n = std::min(n, c.rows.size());
     }
     if (n < 2) throw std::runtime_error("need at least 2 draws per chain");
-    // column selection: sampler columns, then requested (or all) model columns
-    std::vector<std::string> unknown;
-    for (auto& inc : include) {
-      bool found = false;
-      for (auto& nm : names) {
-        const std::string dn = display_name(nm);
-        found = found || nm == inc || dn == inc || dn.rfind(inc + "[", 0) == 0;
-      }
-      if (!found) unknown.push_back(inc);
-    }
-    if (!unknown.empty()) {
-      std::cout << "--include_param: Unrecognized parameter(s): ";
-      for (auto& u : unknown) std::cout << "'" << u << "' ";
-      std::cout << std::endl;
-      return 1;
-    }
+    // column selection: lp__ and the sampler columns always; the model
+    // columns all, or those whose display name (theta[1]; a request written
+    // theta.1 is converted first) equals a requested one exactly. Requests
+    // are a set; any left unmatched are reported in sorted order
+    // (stansummary.cpp:158-195).
+    auto is_sampler_name = [](const std::string& nm) {
+      return nm.size() >= 2 && nm.compare(nm.size() - 2, 2, "__") == 0;
+    };
+    std::set<std::string> requested;
+    for (auto& inc : include) requested.insert(display_name(inc));
     std::vector<int> cols;
     for (size_t j = 0; j < names.size(); ++j) {
-      const bool sampler = names[j].size() > 2 && names[j].compare(names[j].size() - 2, 2, "__") == 0;
-      bool keep = include.empty() || sampler;
-      for (auto& inc : include) {
-        const std::string dn = display_name(names[j]);
-        if (names[j] == inc || dn == inc || dn.rfind(inc + "[", 0) == 0) keep = true;
-      }
-      if (keep) cols.push_back((int)j);
+      if (is_sampler_name(names[j])) cols.push_back((int)j);
+      else if (include.empty() || requested.erase(display_name(names[j])) > 0) cols.push_back((int)j);
+    }
+    if (!requested.empty()) {
+      std::cout << "--include_param: Unrecognized parameter(s): ";
+      for (auto& u : requested) std::cout << "'" << u << "' ";
+      std::cout << std::endl;
+      return 1;
     }
     const int P = (int)cols.size(), M = (int)chains.size();
     std::vector<double> dr((size_t)P * M * n);
@@ -270,8 +283,8 @@ int main(int argc, char** argv) {
     // formats per column are chosen separately for the sampler rows and the
     // model rows (lp__ prints with the model rows' formats); a column is as
     // wide as the wider of the two (stansummary.cpp:205-246)
-    size_t wname = 0;
-    for (int p = 0; p < P; ++p) wname = std::max(wname, display_name(names[cols[p]]).size());
+    size_t wname = 0;  // over every column of the file, reported or not
+    for (auto& nm : names) wname = std::max(wname, display_name(nm).size());
     auto is_sampler = [&](int p) {
       const std::string& nm = names[cols[p]];
       return nm != "lp__" && nm.size() > 2 && nm.compare(nm.size() - 2, 2, "__") == 0;
@@ -317,20 +330,20 @@ int main(int argc, char** argv) {
     std::cout << "\n";
     write_sampler_info("", std::cout);
     if (autocorr > 0) {
-      // normalised autocorrelation of every reported column of one chain,
-      // lags 0 .. n-1 (stansummary_helper.hpp:700-745 layout)
+      // normalised autocorrelation of every column of one chain (lp__, the
+      // sampler columns and all model columns, whatever --include_param
+      // selects), lags 0 .. n-1, under the display names
+      // (stansummary_helper.hpp:700-745 layout)
       const auto& ch = chains[autocorr - 1];
       std::cout << "Displaying the autocorrelations for chain " << autocorr << ":" << std::endl << std::endl;
-      const int nd = (int)n;
+      const int nd = (int)n, NC = (int)names.size();
       int lag_w = 1;
       for (int x = nd; x != 0; x /= 10) ++lag_w;
-      size_t wn = 0;
-      for (int p = 0; p < P; ++p) wn = std::max(wn, names[cols[p]].size());
-      std::vector<std::vector<double>> ac(P, std::vector<double>(nd, 0.0));
-      for (int p = 0; p < P; ++p) {
-        double m = 0.0, lo = ch.rows[0][cols[p]], hi = lo;
+      std::vector<std::vector<double>> ac(NC, std::vector<double>(nd, 0.0));
+      for (int p = 0; p < NC; ++p) {
+        double m = 0.0, lo = ch.rows[0][p], hi = lo;
         for (int k = 0; k < nd; ++k) {
-          const double x = ch.rows[k][cols[p]];
+          const double x = ch.rows[k][p];
           m += x;
           lo = std::min(lo, x);
           hi = std::max(hi, x);
@@ -341,7 +354,7 @@ int main(int argc, char** argv) {
           continue;
         }
         std::vector<double> d(nd);
-        for (int k = 0; k < nd; ++k) d[k] = ch.rows[k][cols[p]] - m;
+        for (int k = 0; k < nd; ++k) d[k] = ch.rows[k][p] - m;
         double c0 = 0.0;
         for (int k = 0; k < nd; ++k) c0 += d[k] * d[k];
         for (int lag = 0; lag < nd; ++lag) {
@@ -352,11 +365,11 @@ int main(int argc, char** argv) {
       }
       std::cout << std::setprecision(6) << std::defaultfloat;
       std::cout << std::setw(lag_w > 4 ? lag_w : 4) << "Lag";
-      for (int p = 0; p < P; ++p) std::cout << std::setw((int)wn + 1) << std::right << names[cols[p]];
+      for (int p = 0; p < NC; ++p) std::cout << std::setw((int)wname + 1) << std::right << display_name(names[p]);
       std::cout << std::endl;
       for (int lag = 0; lag < nd; ++lag) {
         std::cout << std::setw(lag_w) << std::right << lag;
-        for (int p = 0; p < P; ++p) std::cout << std::setw((int)wn + 1) << std::right << ac[p][lag];
+        for (int p = 0; p < NC; ++p) std::cout << std::setw((int)wname + 1) << std::right << ac[p][lag];
         std::cout << std::endl;
       }
     }

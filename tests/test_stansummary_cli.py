@@ -71,8 +71,13 @@ def test_autocorr_option(tmp_path):
 @pytest.mark.parametrize("args,msg", [
     (["--percentiles", "-1"], "Option --percentiles"),           # stansummary_test.cpp:375-406
     (["--percentiles", "101"], "Option --percentiles"),
-    (["--percentiles", "2,30,5"], "Option --percentiles"),
+    (["--percentiles", "2,30,5"], "Option --percentiles 2,30,5: values must be in range (0, 100), inclusive, "
+                                  "and strictly increasing."),
+    (["--percentiles", "5,abc"], "Option --percentiles"),
     (["-i", "psi"], "--include_param: Unrecognized parameter(s): 'psi' "),  # :408-424
+    # lp__ and the sampler columns are not model parameters (stansummary.cpp:158-190)
+    (["-i", "psi", "-i", "lp__", "-i", "accept_stat__"],
+     "--include_param: Unrecognized parameter(s): 'accept_stat__' 'lp__' 'psi' "),
 ])
 def test_bad_arguments_are_errors(args, msg):
     r = subprocess.run([BIN] + args + [CSV], capture_output=True, text=True, timeout=60)
@@ -113,6 +118,50 @@ def test_csv_output_sig_figs_golden(tmp_path):
     assert lines[1] == '"lp__",-7.3,0.037,0.77,0.3,-9,-7,-6.8,5.2e+02,5e+02,1'
     assert lines[7] == '"energy__",7.8,0.051,1,0.75,6.8,7.5,9.9,4.9e+02,4.9e+02,1'
     assert lines[8] == '"theta",0.26,0.0061,0.12,0.12,0.08,0.25,0.47,3.6e+02,4e+02,1'
+
+
+def test_percentile_separators_and_equal_values():
+    """The spec splits on commas and spaces alike, runs of them count once, and
+    a value may equal the integer part of the one before
+    (stansummary.cpp:98-110, stansummary_helper.hpp:300-320)."""
+    for spec in ("5 50 95", " 5, 50,,95 ", "5,5,95", "5.5,5.2"):
+        r = subprocess.run([BIN, "--percentiles", spec, CSV], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, (spec, r.stdout)
+        head = [l for l in r.stdout.splitlines() if "Mean" in l][0].split()
+        want = [t for t in spec.replace(",", " ").split()]
+        assert head[4:4 + len(want)] == [t + "%" for t in want], (spec, head)
+
+
+def test_existing_csv_path_is_refused(tmp_path):
+    """--csv_filename must name a path that does not exist yet (the CLI11
+    NonexistentPath check of stansummary.cpp:62-64)."""
+    out = tmp_path / "there.csv"
+    out.write_text("keep me\n")
+    r = subprocess.run([BIN, f"--csv_filename={out}", CSV], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Path already exists" in r.stdout
+    assert out.read_text() == "keep me\n"
+
+
+def test_include_param_is_exact_and_autocorr_shows_every_column():
+    """A vector's bare name matches none of its elements; the console rows use
+    display names; --autocorr prints every column (lp__, sampler, all model
+    parameters) under display names whatever --include_param selects
+    (stansummary_helper.hpp:700-745)."""
+    es = os.path.join(ROOT, "tests", "golden", "eight_schools_output.csv")
+    r = subprocess.run([BIN, "-i", "theta", es], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Unrecognized parameter(s): 'theta' " in r.stdout
+    r = subprocess.run([BIN, "-a", "1", "-i", "theta.2", "-i", "mu", es], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    body, ac = r.stdout.split("Displaying the autocorrelations for chain 1:")
+    rows = [l.split()[0] for l in body.splitlines() if l and not l.startswith(" ")]
+    assert "theta[2]" in rows and "mu" in rows and "theta[1]" not in rows and "tau" not in rows
+    head = ac.strip().splitlines()[0].split()
+    assert head == ["Lag", "lp__", "accept_stat__", "stepsize__", "treedepth__", "n_leapfrog__", "divergent__",
+                    "energy__", "mu", "tau"] + [f"theta[{k}]" for k in range(1, 9)]
+    # every name column is as wide as the longest display name plus one
+    lines = [l for l in ac.splitlines() if l.strip()]
+    lag_w = max(len(str(len(lines) - 1)) + 1, 4)  # lags 0 .. n-1; the Lag column is one wider than n's digits
+    assert lines[0] == "Lag".rjust(lag_w) + "".join(h.rjust(len("accept_stat__") + 1) for h in head[1:])
 
 
 def test_csv_output_include_param_golden(tmp_path):

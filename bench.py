@@ -41,6 +41,23 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix
+
+
+def reduced_kernel(precision, path, D):
+    """(kernel name, matrix-core peak TFLOP/s, matrix products per algorithmic
+    multiply-add) of the reduced-precision logistic-regression gradient the
+    autotuner picked: path 0 single-pass FP32 cores; path 1 two-pass (BF16
+    cores with three-term operands when X is stored in BF16 and D % 8 == 0,
+    else FP32 cores); path 2 single-pass BF16 cores with three-term operands
+    (six products with X in FP32, three with X stored in BF16)."""
+    if path == 2:
+        return "glm_mfma_bf16", PEAK_BF16_TFLOPS, 6 if precision == "fp32" else 3
+    if path == 1:
+        if precision == "bf16" and D % 8 == 0 and os.environ.get("CMDSTAN_AMD_GL_BF16", "1") != "0":
+            return "gbl_eta_kernel + gbl_grad_kernel", PEAK_BF16_TFLOPS, 3
+        return "gl_eta_kernel + gl_grad_kernel", PEAK_FP32_TFLOPS, 1
+    return "glm_mfma_f32", PEAK_FP32_TFLOPS, 1
 
 
 def measured_traffic(kernel, args):
@@ -176,12 +193,10 @@ def learned_leg(model, args, n_draws, plain, plain_samp_s):
     if fit.grad_seconds_per_launch > 0 and fit.grad_chains_per_launch > 0 and not fit.flow_used:
         t = fit.grad_seconds_per_launch
         ach = 4.0 * args.N * D * fit.grad_chains_per_launch / t / 1e12
-        roof = dict(bound="mfma", achieved=ach, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP32_TFLOPS,
-                    kernel={0: "glm_mfma_f32", 1: "gl_eta_kernel + gl_grad_kernel",
-                            2: "glm_mfma_bf16"}.get(fit.tuned_path), launch_ms=t * 1e3,
-                    mean_active_chains=fit.grad_chains_per_launch,
-                    note="FP32-level products; peak is the FP32 matrix peak (the BF16-split kernel runs on the BF16 "
-                         "matrix cores, 6 products per FP32 multiply-add)")
+        kname, peak, nprod = reduced_kernel("fp32", fit.tuned_path, D)
+        roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak, kernel=kname,
+                    launch_ms=t * 1e3, mean_active_chains=fit.grad_chains_per_launch,
+                    products_per_multiply_add=nprod, frac_of_product_ceiling=ach * nprod / peak)
     return dict(
         path="FP32 contraction (X FP32) + affine-coupling flow (8 layers, MLP 2x128) in automatic mode",
         autotuned_gradient=["single-pass", "two-pass", "single-pass bf16-split"][fit.tuned_path]
@@ -379,7 +394,8 @@ def bench_c4(args):
 
 def bench_c5(args):
     """Logistic regression N=1,000,000 x D=1000 (X stored BF16 or FP32), 512
-    chains, dense_e, FP32 matrix-core X.B / R^T.X with FP64 sums."""
+    chains, dense_e; the two products on BF16 matrix cores with three-term
+    operands (X in BF16) or FP32 matrix cores (X in FP32), FP64 sums."""
     import cmdstan_amd as ca
     from cmdstan_amd.summary import summarize
     N, D, C = args.N5, args.D5, 512
@@ -401,9 +417,18 @@ def bench_c5(args):
     grads = float(arr[:, :, 4].sum())
     t = fit.grad_seconds_per_launch
     flops = 4.0 * N * D * fit.grad_chains_per_launch
-    roof = dict(bound="mfma", achieved=flops / t / 1e12, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                frac=flops / t / 1e12 / PEAK_FP32_TFLOPS, traffic=None, kernel="gl_eta_kernel + gl_grad_kernel",
+    # X stored in BF16 (D a multiple of 8) runs on BF16 matrix cores with the
+    # other operand as three BF16 terms: three BF16 products per algorithmic
+    # multiply-add, so the ceiling at this accuracy is a third of the BF16 peak
+    bf16_cores = prec == "bf16" and D % 8 == 0 and os.environ.get("CMDSTAN_AMD_GL_BF16", "1") != "0"
+    peak = PEAK_BF16_TFLOPS if bf16_cores else PEAK_FP32_TFLOPS
+    roof = dict(bound="mfma", achieved=flops / t / 1e12, peak=peak, unit="TFLOP/s",
+                frac=flops / t / 1e12 / peak, traffic=None,
+                kernel="gbl_eta_kernel + gbl_grad_kernel" if bf16_cores else "gl_eta_kernel + gl_grad_kernel",
                 launch_ms=t * 1e3, mean_active_chains=fit.grad_chains_per_launch) if t > 0 else None
+    if roof and bf16_cores:
+        roof["products_per_multiply_add"] = 3
+        roof["frac_of_three_product_ceiling"] = roof["frac"] * 3
     return {"metric": "min bulk-ESS/sec, large logistic regression (dense_e)", "value": ess / fit.sampling_seconds,
             "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": fit.sampling_seconds / args.steps * 1e3, "higher_is_better": True,
@@ -489,13 +514,17 @@ def main():
         t = fit.grad_seconds_per_launch
         flops_per_launch = 4.0 * args.N * D * fit.grad_chains_per_launch
         ach = flops_per_launch / t / 1e12
-        peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
-        kname = ("glm_mfma_f64" if args.precision == "fp64" else
-                 {0: "glm_mfma_f32", 1: "gl_eta_kernel + gl_grad_kernel", 2: "glm_mfma_bf16"}.get(fit.tuned_path))
+        if args.precision == "fp64":
+            kname, peak, nprod = "glm_mfma_f64", PEAK_FP64_TFLOPS, 1
+        else:
+            kname, peak, nprod = reduced_kernel(args.precision, fit.tuned_path, D)
         roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
                     traffic=measured_traffic(kname, args), kernel=kname, launch_ms=t * 1e3,
                     launches=fit.grad_launches_timed, mean_active_chains=fit.grad_chains_per_launch,
                     flops_per_launch=flops_per_launch)
+        if nprod > 1:
+            roof["products_per_multiply_add"] = nprod
+            roof["frac_of_product_ceiling"] = ach * nprod / peak
     learned = None
     if not args.no_learned_leg:
         args._plain_warm_s = fit.warmup_seconds

@@ -166,7 +166,52 @@ struct ModelImpl {
     }
     return xlow.p;
   }
+  // BF16 X transposed [Dp][ldn] for pass 2 of the BF16 large-D kernels (made
+  // once from the BF16 X, kept beside it: both fit HBM many times over)
+  DevBuf<unsigned short> xt;
+  const unsigned short* bf16_xt(hipStream_t st) {
+    if (!xt.p) {
+      const GlLargePlan gp = gl_large_plan(glmN, 1, glmD, 0);
+      const unsigned short* xb = static_cast<const unsigned short*>(low_x(2, st));
+      xt.alloc(gp.xt_elems);
+      gl_large_prepare_xt(xb, glmN, glmD, xt.p, gp, st);
+      HIPC(hipStreamSynchronize(st));
+    }
+    return xt.p;
+  }
 };
+
+// Scratch of the large-D two-pass gradient.  With X stored in BF16 (and D a
+// multiple of 8) the products run on BF16 matrix cores with positions and
+// residuals as three BF16 terms; CMDSTAN_AMD_GL_BF16=0 keeps them on FP32
+// matrix cores.
+struct GlScratch {
+  DevBuf<float> qf, r;
+  DevBuf<unsigned short> qb, rb;
+  DevBuf<double> wg, wlp;
+};
+bool gl_bf16_enabled(int precision, int D) {
+  if (precision != 2 || !gl_large_bf16_ok(D)) return false;
+  const char* e = std::getenv("CMDSTAN_AMD_GL_BF16");
+  return !e || std::atoi(e) != 0;
+}
+void gl_bind(ModelImpl& M, GlLargeArgs& a, const GlLargePlan& p, GlScratch& s, int precision, hipStream_t st) {
+  s.wg.alloc(p.wg_doubles);
+  s.wlp.alloc(p.wlp_doubles);
+  a.y = M.d_y; a.N = M.glmN; a.D = M.glmD; a.s2inv = M.glm_s2inv;
+  a.Wg = s.wg.p; a.Wlp = s.wlp.p; a.Dq = p.Dq; a.ldr = p.ldr;
+  if (gl_bf16_enabled(precision, M.glmD)) {
+    s.qb.alloc(p.qb_elems);
+    s.rb.alloc(p.rb_elems);
+    a.Qb = s.qb.p; a.Rb = s.rb.p; a.Xt = M.bf16_xt(st);
+    a.Qf = nullptr; a.R = nullptr;
+  } else {
+    s.qf.alloc(p.q_floats);
+    s.r.alloc(p.r_floats);
+    a.Qf = s.qf.p; a.R = s.r.p;
+    a.Qb = nullptr; a.Rb = nullptr; a.Xt = nullptr;
+  }
+}
 
 // Register layout of the 16x16 f32 MFMA result (0: row 4*(l/16)+i, 1: row 4i + l/16).
 constexpr int kF32Layout = 0;
@@ -463,11 +508,10 @@ ModelImpl::GlmTune glm_autotune(ModelImpl& M, int C, int precision, hipStream_t 
       if (ms < best.ms) { best.ms = ms; best.path = 2; best.blocks_per_cu = bpc; }
     }
     const GlLargePlan gp = gl_large_plan(N, C, D, num_cus());
-    DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
-    DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
+    GlScratch gs;
     GlLargeArgs a;
-    a.y = M.d_y; a.N = N; a.D = D; a.C = C; a.s2inv = M.glm_s2inv; a.Q = Q.p; a.ldq = Dp; a.G = G.p; a.V = V.p;
-    a.Qf = qf.p; a.Dq = gp.Dq; a.R = rr.p; a.ldr = gp.ldr; a.Wg = wg.p; a.Wlp = wl.p;
+    gl_bind(M, a, gp, gs, precision, st);
+    a.C = C; a.Q = Q.p; a.ldq = Dp; a.G = G.p; a.V = V.p;
     const double ms = time_it([&]() { gl_large_grad(a, gp, xl, precision, st); });
     if (ms < best.ms) { best.ms = ms; best.path = 1; }
   }
@@ -767,16 +811,13 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   }
   GlLargeArgs gla;
   GlLargePlan glp;
-  DevBuf<float> glQf, glR;
-  DevBuf<double> glWg, glWlp;
+  GlScratch gls;
   if (!fused && gl_ok) {
     act_idx.alloc(C);
     n_act.alloc(1);
     glp = gl_large_plan(M.glmN, C, M.glmD, num_cus());
-    glQf.alloc(glp.q_floats); glR.alloc(glp.r_floats); glWg.alloc(glp.wg_doubles); glWlp.alloc(glp.wlp_doubles);
-    gla.y = M.d_y; gla.N = M.glmN; gla.D = M.glmD; gla.C = C; gla.s2inv = M.glm_s2inv;
-    gla.Q = Q.p; gla.ldq = Dp[0]; gla.G = G.p; gla.V = V.p; gla.idx = act_idx.p; gla.n_active = n_act.p;
-    gla.Qf = glQf.p; gla.Dq = glp.Dq; gla.R = glR.p; gla.ldr = glp.ldr; gla.Wg = glWg.p; gla.Wlp = glWlp.p;
+    gl_bind(M, gla, glp, gls, cfg.precision, st);
+    gla.C = C; gla.Q = Q.p; gla.ldq = Dp[0]; gla.G = G.p; gla.V = V.p; gla.idx = act_idx.p; gla.n_active = n_act.p;
     if (R.flow) {
       gla.Q = fX.p; gla.ldq = D[0]; gla.G = fGX.p; gla.V = fVX.p; gla.idx = ident.p;
     }
@@ -1457,11 +1498,10 @@ static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n
       HIPC(hipDeviceSynchronize());
     } else if (M.glm_large && dataset == 0 && precision != 0) {
       const GlLargePlan gp = gl_large_plan(M.glmN, C, D, num_cus());
-      DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
-      DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
+      GlScratch gs;
       GlLargeArgs a;
-      a.y = M.d_y; a.N = M.glmN; a.D = D; a.C = C; a.s2inv = M.glm_s2inv; a.Q = Q.p; a.ldq = Dp; a.G = G.p; a.V = V.p;
-      a.Qf = qf.p; a.Dq = gp.Dq; a.R = rr.p; a.ldr = gp.ldr; a.Wg = wg.p; a.Wlp = wl.p;
+      gl_bind(M, a, gp, gs, precision, 0);
+      a.C = C; a.Q = Q.p; a.ldq = Dp; a.G = G.p; a.V = V.p;
       gl_large_grad(a, gp, M.low_x(precision, 0), precision, 0);
       HIPC(hipDeviceSynchronize());
     } else if (M.glm && dataset == 0) {
@@ -1469,11 +1509,10 @@ static int log_prob_grad_impl(nuts_model* m, int dataset, const double* q, int n
       const ModelImpl::GlmTune tn = precision != 0 ? glm_autotune(M, C, precision, 0) : ModelImpl::GlmTune{};
       if (tn.path == 1) {
         const GlLargePlan gp = gl_large_plan(M.glmN, C, D, num_cus());
-        DevBuf<float> qf(gp.q_floats), rr(gp.r_floats);
-        DevBuf<double> wg(gp.wg_doubles), wl(gp.wlp_doubles);
+        GlScratch gs;
         GlLargeArgs a;
-        a.y = M.d_y; a.N = M.glmN; a.D = D; a.C = C; a.s2inv = M.glm_s2inv; a.Q = Q.p; a.ldq = Dp; a.G = G.p;
-        a.V = V.p; a.Qf = qf.p; a.Dq = gp.Dq; a.R = rr.p; a.ldr = gp.ldr; a.Wg = wg.p; a.Wlp = wl.p;
+        gl_bind(M, a, gp, gs, precision, 0);
+        a.C = C; a.Q = Q.p; a.ldq = Dp; a.G = G.p; a.V = V.p;
         gl_large_grad(a, gp, M.low_x(precision, 0), precision, 0);
       } else {
         const GlmPlan gp = glm_mfma_plan(M.glmN, C, D, num_cus());

@@ -250,19 +250,32 @@ GlLargePlan gl_large_plan(int N, int C, int D, int num_cus) {
   const int tiles2 = ((C + GL_BM - 1) / GL_BM) * ((D + GL_BN - 1) / GL_BN);
   int splits = (2 * (num_cus > 0 ? num_cus : 256) + tiles2 - 1) / tiles2;
   if (splits < 1) splits = 1;
-  int rps = (N + splits - 1) / splits;
-  rps = (rps + GL_BK - 1) / GL_BK * GL_BK;
+  p.ldn = (int64_t)p.row_tiles * GL_BM;
+  // row ranges in whole 64-row slabs (the BF16 variant's K step); a range past
+  // N contributes zero partials
+  int rps = (int)((p.ldn + splits - 1) / splits);
+  rps = (rps + 63) / 64 * 64;
   p.rows_per_split = rps;
-  p.splits = (N + rps - 1) / rps;
+  p.splits = (int)((p.ldn + rps - 1) / rps);
   p.ldr = (C + 3) / 4 * 4;
   p.r_floats = (size_t)N * p.ldr;
   p.q_floats = (size_t)C * p.Dq;
   p.wg_doubles = (size_t)p.splits * C * D;
   p.wlp_doubles = (size_t)p.row_tiles * C;
+  p.Cp = (C + GL_BM - 1) / GL_BM * GL_BM;
+  p.Dk = (D + 63) / 64 * 64;
+  p.Dp = (D + GL_BN - 1) / GL_BN * GL_BN;
+  p.qb_elems = (size_t)3 * p.Cp * p.Dk;
+  p.rb_elems = (size_t)3 * p.Cp * p.ldn;
+  p.xt_elems = (size_t)p.Dp * p.ldn;
   return p;
 }
 
 void gl_large_grad(const GlLargeArgs& a, const GlLargePlan& p, const void* X, int xtype, hipStream_t st) {
+  if (xtype == 2 && a.Qb && a.Rb && a.Xt) {
+    gl_large_grad_bf16(a, p, static_cast<const unsigned short*>(X), st);
+    return;
+  }
   hipLaunchKernelGGL(gl_pack_q, dim3((unsigned)(((int64_t)a.C * p.Dq + 255) / 256)), dim3(256), 0, st, a);
   if (a.ev_begin) (void)hipEventRecord(a.ev_begin, st);
   const dim3 g1(p.row_tiles, (a.C + GL_BN - 1) / GL_BN);
@@ -276,6 +289,10 @@ void gl_large_grad(const GlLargeArgs& a, const GlLargePlan& p, const void* X, in
                        p.rows_per_split);
   }
   if (a.ev_end) (void)hipEventRecord(a.ev_end, st);
+  gl_reduce_launch(a, p, st);
+}
+
+void gl_reduce_launch(const GlLargeArgs& a, const GlLargePlan& p, hipStream_t st) {
   hipLaunchKernelGGL(gl_reduce, dim3(a.C), dim3(128), 0, st, a, p.splits, p.row_tiles);
 }
 

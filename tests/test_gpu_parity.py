@@ -459,6 +459,57 @@ def test_glm_large_gradient_reduced_precision(precision):
     assert err_g < 1e-5 and err_lp < 1e-5
 
 
+@pytest.mark.parametrize("variant", ["0", "1"])
+@pytest.mark.parametrize("N,D,C", [(6001, 304, 70), (1000, 1000, 130), (3000, 8, 300), (2500, 96, 257)])
+def test_glm_large_bf16_storage_both_kernels(monkeypatch, variant, N, D, C):
+    """X stored in BF16 through the two-pass gradient on FP32 matrix cores
+    (variant 0) and on BF16 matrix cores with positions and residuals as three
+    BF16 terms and the transposed X (variant 1); D <= 128 forced onto the
+    two-pass path.  Error against FP64 on the stored X relative to the sum of
+    |terms|, tolerance 1e-5; ragged N and C, several chain and coefficient
+    tiles, the smallest D the BF16 kernels take (8)."""
+    monkeypatch.setenv("CMDSTAN_AMD_GL_BF16", variant)
+    if D <= 128:
+        monkeypatch.setenv("CMDSTAN_AMD_GLM_PATH", "1")
+    rng = np.random.default_rng(N + D + C)
+    X = rng.standard_normal((N, D))
+    beta = rng.normal(0, 1 / np.sqrt(D), D)
+    y = (rng.random(N) < 1 / (1 + np.exp(-X @ beta))).astype(np.int64)
+    gm = ca.Model("logistic_regression", dict(N=N, D=D, X=X, y=y))
+    Q = beta + rng.normal(0, 0.05, (C, D))
+    print(f"bf16 storage N={N} D={D} C={C} variant={variant}", flush=True)
+    lp, g = gm.log_prob_grad(Q, precision="bf16")
+    Xs = _bf16_round(X)
+    E_ = Xs @ Q.T
+    R = y[:, None] - 1 / (1 + np.exp(-E_))
+    g_ref = (Xs.T @ R).T - Q / 6.25
+    lp_ref = (y[:, None] * E_ - np.logaddexp(0, E_)).sum(0) - 0.5 * (Q * Q).sum(1) / 6.25
+    cond_g = (np.abs(Xs).T @ np.abs(R)).T + np.abs(Q) / 6.25
+    cond_lp = np.abs(y[:, None] * E_).sum(0) + np.logaddexp(0, E_).sum(0)
+    err_g = np.max(np.abs(g - g_ref) / cond_g)
+    err_lp = np.max(np.abs(lp - lp_ref) / cond_lp)
+    print(f"  max rel gradient error {err_g:.2e}, lp error {err_lp:.2e}")
+    assert err_g < 1e-5 and err_lp < 1e-5
+
+
+def test_glm_large_bf16_kernels_sample_the_fp64_posterior():
+    """Sampling through the BF16 matrix-core large-D gradient (X exactly
+    representable in BF16, so FP64 and BF16 storage hold the same data) agrees
+    with the FP64 per-chain path: means within 4 combined MCSE."""
+    N, D = 2000, 136
+    rng = np.random.default_rng(8)
+    X = _bf16_round(rng.standard_normal((N, D)))
+    beta = rng.normal(0, 1 / np.sqrt(D), D)
+    y = (rng.random(N) < 1 / (1 + np.exp(-X @ beta))).astype(np.int64)
+    m = ca.Model("logistic_regression", dict(N=N, D=D, X=X, y=y))
+    a = m.sample(num_chains=32, num_warmup=250, num_samples=150, seed=4, precision="bf16")
+    b = m.sample(num_chains=32, num_warmup=250, num_samples=150, seed=5)
+    assert np.all(a.status == 0) and np.all(b.status == 0)
+    for j in (1, 60, 136):
+        sa, sb = E.summary(a.param(f"beta.{j}")), E.summary(b.param(f"beta.{j}"))
+        assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"]), (j, sa, sb)
+
+
 def test_glm_large_dense_posterior_matches_fp64():
     """dense_e sampling through the large-D FP32 path agrees with the FP64
     per-chain path (means within 4 combined MCSE)."""

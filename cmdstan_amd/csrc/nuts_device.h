@@ -125,6 +125,85 @@ __device__ __forceinline__ void pass_leaf_tail(VT* __restrict__ p, VT* __restric
   }
 }
 
+// unit/diag: end_update_p(eps/2) of a leaf with the kinetic energy's partial,
+// and in the same pass the partial sums of the generalized U-turn checks of
+// the first F merges that close at this leaf (levels 0..F-1: begin lb[k], end
+// le[k], rho lr[k] of the open left siblings; the closing subtree's begin is
+// the leaf for level 0 and lb[k-1] above).  The leaf's momentum goes to dp and
+// the merged rho to dr -- where they are read next -- with optional copies to
+// db (the leaf as its own subtree's begin) and edge (first leaf of a doubling).
+// Every sum follows the separate passes term for term.
+template <int TPC, bool UNIT, int F, class VT>
+__device__ __forceinline__ void pass_leaf_merge(const VT* __restrict__ p, const VT* __restrict__ g,
+                                                const VT* __restrict__ minv, const VT* const (&lb)[3],
+                                                const VT* const (&le)[3], const VT* const (&lr)[3],
+                                                VT* __restrict__ dp, VT* __restrict__ dr, VT* __restrict__ db,
+                                                VT* __restrict__ edge, int D, int tid, double h, double* acc) {
+  double kin = 0;
+  double a[F > 0 ? F : 1][6];
+#pragma unroll
+  for (int k = 0; k < F; ++k)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) a[k][j] = 0;
+#pragma unroll 1
+  for (int i = tid; i < D; i += TPC) {
+    const VT pv = (double)p[i] - h * (double)g[i];
+    const double pd = pv;
+    const double m = UNIT ? 1.0 : (double)minv[i];
+    const double sp = UNIT ? pd : m * pd;
+    kin += pd * sp;
+    double r = pd;  // rho of the closing subtree, as stored (VT)
+    double sbv = pd, sbs = sp;
+#pragma unroll
+    for (int k = 0; k < F; ++k) {
+      const double l = lr[k][i], lbv = lb[k][i], lev = le[k][i];
+      const VT rsv = l + r;
+      const double rs = rsv;
+      const double r2 = l + sbv;
+      const double r3 = r + lev;
+      const double slb = UNIT ? lbv : m * lbv;
+      const double sle = UNIT ? lev : m * lev;
+      a[k][0] += slb * rs; a[k][1] += sp * rs;
+      a[k][2] += slb * r2; a[k][3] += sbs * r2;
+      a[k][4] += sle * r3; a[k][5] += sp * r3;
+      r = rs;
+      sbv = lbv;
+      sbs = slb;
+    }
+    dp[i] = pv;
+    dr[i] = (VT)r;
+    if (db) db[i] = pv;
+    if (edge) edge[i] = pv;
+  }
+  acc[0] = kin;
+#pragma unroll
+  for (int k = 0; k < F; ++k)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[1 + 6 * k + j] = a[k][j];
+}
+
+// unit/diag: the leaf's (q, g, p) proposal slot when it is kept, then
+// begin_update_p(eps/2) and update_q(eps) of the next leaf from the closed
+// subtree's end momentum po
+template <int TPC, bool UNIT, bool SLOT, class VT>
+__device__ __forceinline__ void pass_leaf_next(const VT* __restrict__ po_src, VT* __restrict__ p, VT* __restrict__ q,
+                                               const VT* __restrict__ g, const VT* __restrict__ minv,
+                                               VT* __restrict__ sq, VT* __restrict__ sg, VT* __restrict__ sp, int D,
+                                               int tid, double h, double e) {
+#pragma unroll 4
+  for (int i = tid; i < D; i += TPC) {
+    const VT po = po_src[i], qo = q[i], gi = g[i];
+    if (SLOT) {
+      sq[i] = qo;
+      sg[i] = gi;
+      if (sp) sp[i] = po;
+    }
+    const VT pn = (double)po - h * (double)gi;
+    p[i] = pn;
+    q[i] = (double)qo + e * ((UNIT ? 1.0 : (double)minv[i]) * (double)pn);
+  }
+}
+
 // the three generalized U-turn checks of a merge inside a subtree (rho of the
 // merged subtree written back into srho): partial dot products per thread
 template <int TPC, int METRIC, class VT>
@@ -332,7 +411,7 @@ struct NutsWave {
   }
 
   __device__ VT* vecp(int id) const { return blk + (int64_t)id * Dp; }
-  __device__ VT* Lbeg(int k) const { return vecp(PV_FIXED + k); }
+  __device__ VT* Lbeg(int k) const { return vecp(PV_FIXED + s.L_beg_vec[k]); }
   __device__ VT* Lend(int k) const { return vecp(PV_FIXED + P.maxd + k); }
   __device__ VT* Lrho(int k) const { return vecp(PV_FIXED + 2 * P.maxd + k); }
   __device__ VT* Sq(int k) const { return vecp(PV_FIXED + 3 * P.maxd + k); }
@@ -593,6 +672,7 @@ struct NutsWave {
     // (a transition that ended inside a subtree leaves its checkpoints behind)
     const uint32_t all = P.nslots >= 32 ? 0xFFFFFFFFu : ((1u << P.nslots) - 1u);
     s.free_mask = all & ~(1u << k);
+    for (int l = 0; l < P.maxd; ++l) s.L_beg_vec[l] = (uint8_t)l;
     VT* p = vecp(PV_P);
     VT* ps = vecp(PV_PS);
     copy(q, Sq(k));
@@ -649,8 +729,118 @@ struct NutsWave {
     return request(PH_LEAF);
   }
 
-  // build_tree base case plus the merges that close at this leaf
+  // build_tree base case plus the merges that close at this leaf (unit/diag).
+  // One pass completes the leapfrog, the kinetic energy and the U-turn sums of
+  // up to three of the merges that close here, and stores the leaf's momentum
+  // and merged rho where they are read next: in the open level K they close
+  // into (K = the trailing one bits of the leaf index), or in p and srho at
+  // the doubling's last leaf.  When a check fails, the transition ends and
+  // those stores are never read.
   __device__ bool leaf() {
+    if constexpr (DENSE) {
+      return leaf_dense();
+    } else {
+      constexpr bool UNIT = METRIC == METRIC_UNIT;
+      constexpr int FMAX = 3;
+      const double e = s.dir * s.eps;
+      s.Vz = pending_V();
+      s.n_leapfrog++;
+      const int K = __builtin_ctz(~(unsigned)s.leaf_idx);
+      const bool last = s.leaf_idx == (1 << s.depth) - 1;
+      VT* p = vecp(PV_P);
+      VT* dp = last ? p : Lend(K);
+      VT* dr = last ? vecp(PV_SRHO) : Lrho(K);
+      VT* db = (!last && K == 0) ? Lbeg(0) : nullptr;
+      VT* edge = s.leaf_idx == 0 ? (s.z_is_fwd ? vecp(PV_PFB) : vecp(PV_PBF)) : nullptr;
+      const double hh = 0.5 * e;
+      const int F = K < FMAX ? K : FMAX;
+      const VT* lb[FMAX];
+      const VT* le[FMAX];
+      const VT* lr[FMAX];
+#pragma unroll
+      for (int k = 0; k < FMAX; ++k) {
+        const int kk = k < F ? k : 0;
+        lb[k] = Lbeg(kk);
+        le[k] = Lend(kk);
+        lr[k] = Lrho(kk);
+      }
+      double acc[1 + 6 * FMAX];
+      switch (F) {
+        case 0: pass_leaf_merge<TPC, UNIT, 0>(p, g, minv, lb, le, lr, dp, dr, db, edge, D, tid, hh, acc); break;
+        case 1: pass_leaf_merge<TPC, UNIT, 1>(p, g, minv, lb, le, lr, dp, dr, db, edge, D, tid, hh, acc); break;
+        case 2: pass_leaf_merge<TPC, UNIT, 2>(p, g, minv, lb, le, lr, dp, dr, db, edge, D, tid, hh, acc); break;
+        default: pass_leaf_merge<TPC, UNIT, 3>(p, g, minv, lb, le, lr, dp, dr, db, edge, D, tid, hh, acc); break;
+      }
+      converge();
+      double h = 0.5 * gsum(acc[0]) + s.Vz;
+      if (isnan(h)) h = __builtin_inf();
+      if ((h - s.H0) > P.max_deltaH) s.divergent = 1;
+      s.max_abs_dH = fmax(s.max_abs_dH, fabs(h - s.H0));
+      const double lw = s.H0 - h;
+      if (s.H0 - h > 0) s.sum_metro += 1;
+      else s.sum_metro += exp(s.H0 - h);
+      s.leaf_h = h;
+      if (s.divergent) { trans_end(); return false; }
+      s.S_logw = lw;
+      s.s_prop = PROP_LEAF;
+      int k = 0;
+      while ((s.leaf_idx >> k) & 1) {
+        const double lsw_sub = log_sum_exp2(s.L_logw[k], s.S_logw);
+        bool take;
+        if (s.S_logw > lsw_sub) take = true;
+        else take = uniform() < exp(s.S_logw - lsw_sub);
+        if (take) free_slot(s.L_prop[k]);
+        else { free_slot(s.s_prop); s.s_prop = s.L_prop[k]; }
+        double a1, b1, a2, b2, a3, b3;
+        if (k < F) {
+          const double* ak = acc + 1 + 6 * k;
+          a1 = gsum(ak[0]); b1 = gsum(ak[1]); a2 = gsum(ak[2]);
+          b2 = gsum(ak[3]); a3 = gsum(ak[4]); b3 = gsum(ak[5]);
+        } else {
+          // deeper merges: the separate pass over the stored leaf momentum and
+          // the rho merged so far (k >= F >= 1, so the begin is level k-1's)
+          double am[6];
+          pass_merge<TPC, METRIC, VT>(Lbeg(k), Lend(k), Lrho(k), Lbeg(k - 1), dp, dr, minv, nullptr, nullptr,
+                                      nullptr, nullptr, D, tid, am);
+          converge();
+          a1 = gsum(am[0]); b1 = gsum(am[1]); a2 = gsum(am[2]);
+          b2 = gsum(am[3]); a3 = gsum(am[4]); b3 = gsum(am[5]);
+        }
+        const bool ok = (b1 > 0 && a1 > 0) && (b2 > 0 && a2 > 0) && (b3 > 0 && a3 > 0);
+        s.S_logw = lsw_sub;
+        if (!ok) { free_slot(s.s_prop); trans_end(); return false; }
+        ++k;
+      }
+      if (last) { doubling_end(); return false; }
+      // the closed subtree is the open left sibling at level K: its end and rho
+      // are stored there; its begin is level K-1's begin vector (the oldest
+      // subtree merged), taken over by a swap, or the leaf itself (K = 0)
+      if (K > 0) {
+        const uint8_t t = s.L_beg_vec[K];
+        s.L_beg_vec[K] = s.L_beg_vec[K - 1];
+        s.L_beg_vec[K - 1] = t;
+      }
+      const int slot = s.s_prop == PROP_LEAF ? alloc_slot() : -1;
+      VT* sq = slot >= 0 ? Sq(slot) : nullptr;
+      VT* sg = slot >= 0 ? Sg(slot) : nullptr;
+      VT* spp = slot >= 0 && P.diag ? Sp(slot) : nullptr;
+      if (sq) pass_leaf_next<TPC, UNIT, true>(dp, p, q, g, minv, sq, sg, spp, D, tid, hh, e);
+      else pass_leaf_next<TPC, UNIT, false>(dp, p, q, g, minv, sq, sg, spp, D, tid, hh, e);
+      converge();
+      if (slot >= 0) {
+        s.slot_V[slot] = s.Vz;
+        s.slot_H[slot] = s.leaf_h;
+      }
+      s.L_logw[K] = s.S_logw;
+      s.L_prop[K] = slot >= 0 ? slot : s.s_prop;
+      s.leaf_idx++;
+      return request(PH_LEAF);
+    }
+  }
+
+  // build_tree base case and merges for dense_e (the sharp momenta are kept
+  // beside the momenta in every checkpoint)
+  __device__ bool leaf_dense() {
     const double e = s.dir * s.eps;
     s.Vz = pending_V();
     VT* p = vecp(PV_P);

@@ -58,6 +58,10 @@ struct ChainState {
   int adapt_flag, welford_n;
   int init_rej_lp, init_rej_grad;  // initial values rejected: non-finite log density / gradient
   int L_prop[NUTS_MAX_DEPTH];
+  // vector holding the begin momentum of open level k (unit/diag: a subtree
+  // that closes into level k takes over level k-1's begin vector by swapping
+  // these entries instead of copying the vector)
+  uint8_t L_beg_vec[NUTS_MAX_DEPTH];
   double eps, nom_eps, H0, log_w;
   double sum_metro, Vz, Vo, S_logw;
   double leaf_h, ss_H0, sa_counter, sa_sbar;

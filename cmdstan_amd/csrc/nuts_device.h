@@ -305,6 +305,63 @@ __device__ __noinline__ void dense_matvec(const double* __restrict__ M, const VT
   }
 }
 
+// out1 = M x1 and out2 = M x2 in one pass over M (same order of summation as
+// dense_matvec): the leaf's sharp momentum and M g share the read of the metric
+template <int TPC, class VT>
+__device__ __noinline__ void dense_matvec2(const double* __restrict__ M, const VT* __restrict__ x1,
+                                           const VT* __restrict__ x2, VT* __restrict__ out1, VT* __restrict__ out2,
+                                           int D, int tid) {
+  constexpr int RB = TPC >= 256 ? 4 : 8;
+  for (int i0 = 0; i0 < D; i0 += TPC * RB) {
+    double a1[RB], a2[RB];
+    int ii[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      a1[r] = 0.0;
+      a2[r] = 0.0;
+      const int i = i0 + tid + TPC * r;
+      ii[r] = i < D ? i : D - 1;
+    }
+    int j = 0;
+    for (; j + 4 <= D; j += 4) {
+      double v[4][RB], y1[4], y2[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        y1[u] = x1[j + u];
+        y2[u] = x2[j + u];
+        const double* col = M + (int64_t)(j + u) * D;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) v[u][r] = col[ii[r]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          a1[r] += v[u][r] * y1[u];
+          a2[r] += v[u][r] * y2[u];
+        }
+    }
+    for (; j < D; ++j) {
+      const double y1j = x1[j], y2j = x2[j];
+      const double* col = M + (int64_t)j * D;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const double m = col[ii[r]];
+        a1[r] += m * y1j;
+        a2[r] += m * y2j;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int i = i0 + tid + TPC * r;
+      if (i < D) {
+        out1[i] = a1[r];
+        out2[i] = a2[r];
+      }
+    }
+  }
+}
+
 // col += w * d (the dense Welford covariance update, one column per call)
 template <int TPC, class VT>
 __device__ __forceinline__ void pass_axpy(double* __restrict__ col, const VT* __restrict__ w, double d, int D,
@@ -538,6 +595,36 @@ struct NutsWave {
     for (int i = tid; i < D; i += TPC) p[i] = (double)p[i] - h * (double)gr[i];
     converge();
     if (DENSE) matvec(p, vecp(PV_PS));
+  }
+
+  // dense_e leaf: end_update_p(eps/2), then the sharp momentum M p and M g in
+  // one pass over the metric; M g stays in PV_TMP for leap_begin_mg
+  __device__ void leap_end_mg(double e) const {
+    VT* __restrict__ p = vecp(PV_P);
+    const VT* __restrict__ gr = g;
+    const double h = 0.5 * e;
+    for (int i = tid; i < D; i += TPC) p[i] = (double)p[i] - h * (double)gr[i];
+    converge();
+    gsync();
+    dense_matvec2<TPC, VT>(dM, p, g, vecp(PV_PS), vecp(PV_TMP), D, tid);
+    gsync();
+  }
+  // dense_e: begin_update_p(eps/2) and update_q(eps) of the next leaf, with
+  // M (p - h g) = M p - h M g from the products leap_end_mg left (g unchanged
+  // in between): no second pass over the metric
+  __device__ void leap_begin_mg(double e) const {
+    VT* __restrict__ p = vecp(PV_P);
+    const VT* __restrict__ ps = vecp(PV_PS);
+    VT* __restrict__ t = vecp(PV_TMP);
+    VT* __restrict__ qr = q;
+    const VT* __restrict__ gr = g;
+    const double h = 0.5 * e;
+    for (int i = tid; i < D; i += TPC) {
+      p[i] = (double)p[i] - h * (double)gr[i];
+      const double ti = (double)ps[i] - h * (double)t[i];
+      qr[i] = (double)qr[i] + e * ti;
+    }
+    converge();
   }
 
   // ------------------------------------------------------------- slots
@@ -849,7 +936,7 @@ struct NutsWave {
     s.n_leapfrog++;
     double h;
     if (DENSE) {
-      leap_end(e);
+      leap_end_mg(e);
       h = kinetic(p, ps) + s.Vz;
     } else {
       // leap_end, kinetic, the edge copy and the srho copy in one pass (srho
@@ -915,7 +1002,7 @@ struct NutsWave {
       s.L_logw[k] = s.S_logw;
       s.L_prop[k] = s.s_prop == PROP_LEAF ? store_leaf_slot() : s.s_prop;
       s.leaf_idx++;
-      leap_begin(e);
+      leap_begin_mg(e);
       return request(PH_LEAF);
     }
     // unit/diag: the checkpoint copies, the proposal slot and the next leaf's

@@ -2,6 +2,7 @@
 // with split-K and fused bias / tanh / tanh' epilogues, the coupling
 // element-wise steps, and Adam.  Host class FlowNet in flow.h.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <stdexcept>
@@ -888,7 +889,13 @@ static int ff_h1_splits(int Da, int H, int n_rows, int64_t part_n, int* kper) {
   return (Da + *kper - 1) / *kper;
 }
 
+// The fused per-layer kernels run only with CMDSTAN_AMD_FLOW_FUSED=1: at 256
+// rows and D = 4020 they measured 2.7x slower than the split-K GEMM chain
+// (their h1 / GP2 partial sums are reduced by one block per 64 x 64 tile,
+// eight blocks in all), so the GEMM chain is the default.
 bool FlowNet::fused_ok(int n_rows) const {
+  const char* e = std::getenv("CMDSTAN_AMD_FLOW_FUSED");
+  if (!e || std::atoi(e) == 0) return false;
   const int64_t rh = (int64_t)n_rows * spec_.H;
   return spec_.H <= FF_HMAX && (int64_t)((Dmax_b_ + 63) / 64) * rh <= part_n_;
 }

@@ -86,10 +86,13 @@ def test_c4_batched_gradient_256_points():
     assert worst <= 1e-12
 
 
-def test_c4_flow_forward_and_pullback_d4020():
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_c4_flow_forward_and_pullback_d4020(monkeypatch, fused):
     """The C4 flow (8 affine-coupling layers, MLP 2 x 128, masks with the 20
     hyperparameters as globals and groups of 10) at D = 4020, 256 rows, against
-    the FP64 oracle at the FP32 tolerance 2e-4."""
+    the FP64 oracle at the FP32 tolerance 2e-4, on the split-K GEMM chain and
+    on the fused per-layer kernels (CMDSTAN_AMD_FLOW_FUSED=1)."""
+    monkeypatch.setenv("CMDSTAN_AMD_FLOW_FUSED", fused)
     D, L, H, ng, gs = 4020, 8, 128, 20, 10
     g = ca.Flow(D, n_layers=L, hidden=H, n_glob=ng, group_size=gs, max_rows=256, seed=3)
     rng = np.random.default_rng(3)

@@ -178,7 +178,7 @@ def learned_leg(model, args, n_draws, plain, plain_samp_s):
     fit = model.sample(num_chains=args.chains, num_warmup=1000, num_samples=n_draws, seed=args.seed,
                        metric="diag_e", fused=0, precision="fp32", flow="auto", flow_layers=8, flow_hidden=128,
                        flow_train_steps=args.flow_steps, flow_batch=256, flow_warmup=args.flow_warmup,
-                       flow_pilot=args.flow_pilot)
+                       flow_pilot=args.flow_pilot, energy_error=True)
     wall = time.time() - t0
     st = leg_stats(fit, D)
     ok = st["rhat_max"] < 1.01
@@ -392,6 +392,33 @@ def bench_c4(args):
             "speedup_total": out["flow"]["ess_per_sec_total"] / out["plain"]["ess_per_sec_total"]}
 
 
+def bench_c4_flow_kernels(args):
+    """The flow alone at the C4 shape: forward T(z) and the pullback of a
+    gradient for 256 rows, D = 4020, 8 coupling layers, MLP 2 x 128 (the
+    kernels a flow leaf adds to a plain one); wall time per forward + pullback
+    including the host copies of z and the results (a kernel trace gives the
+    GPU time).  CMDSTAN_AMD_FLOW_FUSED=1 selects the fused per-layer kernels."""
+    import cmdstan_amd as ca
+    D, L, H, rows = 4020, 8, 128, 256
+    g = ca.Flow(D, n_layers=L, hidden=H, n_glob=20, group_size=10, max_rows=rows, seed=3)
+    rng = np.random.default_rng(args.seed)
+    z = rng.standard_normal((rows, D))
+    gx = rng.standard_normal((rows, D))
+    for _ in range(3):
+        g.forward(z)
+        g.pullback_grad(z, gx)
+    n = max(1, args.steps)
+    t0 = time.time()
+    for _ in range(n):
+        g.forward(z)
+        g.pullback_grad(z, gx)
+    wall = (time.time() - t0) / n
+    return {"metric": "flow forward + pullback wall time (C4 shape)", "value": wall * 1e3, "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1, "steps": n, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"affine-coupling flow D={D}, {L} layers, MLP 2x{H}, {rows} rows",
+                       "fused_layer_kernels": os.environ.get("CMDSTAN_AMD_FLOW_FUSED", "0") not in ("", "0")}}
+
+
 def bench_c5(args):
     """Logistic regression N=1,000,000 x D=1000 (X stored BF16 or FP32), 512
     chains, dense_e; the two products on BF16 matrix cores with three-term
@@ -453,7 +480,7 @@ def main():
                     help="c2: skip the batched FP32 + flow leg of the metric")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"],
                     help="GLM contraction precision (sums, H and tree arithmetic stay FP64)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c4f", "c5"],
                     help="c2 (default): the headline workload; c1, c3: secondary measurements")
     ap.add_argument("--models", type=int, default=512, help="c3: number of models")
     ap.add_argument("--c3-batched-only", action="store_true", help="c3: skip the model-at-a-time comparison")
@@ -483,6 +510,9 @@ def main():
     if args.config == "c4":
         print(json.dumps(bench_c4(args)))
         return
+    if args.config == "c4f":
+        print(json.dumps(bench_c4_flow_kernels(args)))
+        return
     if args.config == "c5":
         print(json.dumps(bench_c5(args)))
         return
@@ -499,7 +529,7 @@ def main():
           file=sys.stderr, flush=True)
     t0 = time.time()
     fit = model.sample(num_chains=C, num_warmup=n_warm, num_samples=n_draws, seed=args.seed,
-                       metric="diag_e", fused=0, precision=args.precision)
+                       metric="diag_e", fused=0, precision=args.precision, energy_error=True)
     wall = time.time() - t0
     plain = leg_stats(fit, D)
     samp_s = fit.sampling_seconds
@@ -515,7 +545,9 @@ def main():
         flops_per_launch = 4.0 * args.N * D * fit.grad_chains_per_launch
         ach = flops_per_launch / t / 1e12
         if args.precision == "fp64":
-            kname, peak, nprod = "glm_mfma_f64", PEAK_FP64_TFLOPS, 1
+            # even D runs the double-buffered LDS-copy variant
+            kname = "glm_mfma_f64_lds" if D % 2 == 0 else "glm_mfma_f64"
+            peak, nprod = PEAK_FP64_TFLOPS, 1
         else:
             kname, peak, nprod = reduced_kernel(args.precision, fit.tuned_path, D)
         roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,

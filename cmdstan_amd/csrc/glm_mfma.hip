@@ -157,6 +157,149 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
   if (lk == 0 && c0 + lr < n_act) Wlp[(int64_t)chunk * sp.stride + c0 + lr] = lp;
 }
 
+// Even D: the same products on 32-row tiles double-buffered in LDS.  The next
+// tile's rows are copied global -> LDS by global_load_lds_dwordx4 (16 bytes
+// per lane, lane l at row base + 16 l, exec-masked lanes leave their slots
+// alone -- checked on the device), one instruction per row, issued before the
+// current tile's products and waited for (vmcnt, then the barrier) after
+// them: the copy needs no registers and overlaps the matrix work.  Columns
+// past D stay zero from the initial clear; rows past the chunk hold finite
+// data of an earlier tile and contribute nothing (their residual is 0).  Rows
+// are summed in the order of the 64-row kernel, so the products match it bit
+// for bit; the per-observation terms use two reciprocals instead of divisions.
+typedef __attribute__((address_space(3))) void lds_void;
+#define GL_TN 32  // rows per double-buffered tile
+
+template <int KMAX>
+__global__ void __launch_bounds__(256, 2)
+glm_mfma_f64_lds(const double* __restrict__ X, const double* __restrict__ y, int N, int D, int total_blocks,
+                 const double* __restrict__ Q, int64_t ldq, int C, const int* __restrict__ idx,
+                 const int* __restrict__ n_active, double* __restrict__ Wg, double* __restrict__ Wlp) {
+  __shared__ __attribute__((aligned(16))) double sX[2][GL_TN * GM_XS];
+  __shared__ double sY[2][GL_TN];
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  const int l = tid & 63;
+  const int lr = l & 15;
+  const int lk = l >> 4;
+
+  const int n_act = idx ? *n_active : C;
+  const GlmSplit sp = glm_split(n_act, N, total_blocks);
+  const int lin = blockIdx.x;
+  const int xcd = lin & 7;
+  const int j = lin >> 3;
+  const int chunk = xcd + 8 * (j / sp.groups);
+  const int group = j % sp.groups;
+  if (chunk >= sp.chunks || group * GM_TC >= n_act) return;
+
+  const int c0 = group * GM_TC + 16 * w;
+  const int KS = (D + 3) >> 2;
+  const int DT = (D + 15) >> 4;
+  double qreg[KMAX / 4];
+  {
+    const int pos = c0 + lr;
+    const int c = pos < n_act ? (idx ? idx[pos] : pos) : -1;
+#pragma unroll
+    for (int ks = 0; ks < KMAX / 4; ++ks) {
+      const int k = 4 * ks + lk;
+      qreg[ks] = (ks < KS && c >= 0 && k < D) ? Q[(int64_t)c * ldq + k] : 0.0;
+    }
+  }
+  d4 gacc[KMAX / 16];
+#pragma unroll
+  for (int t = 0; t < KMAX / 16; ++t) gacc[t] = d4{0, 0, 0, 0};
+  double lp = 0.0;
+
+  const int n_begin = chunk * sp.rows;
+  const int n_end = min(N, n_begin + sp.rows);
+  for (int i = tid; i < 2 * GL_TN * GM_XS; i += 256) (&sX[0][0])[i] = 0.0;
+  const int nch = D >> 1;  // 16-byte chunks per row
+  // wave w copies rows 8 w .. 8 w + 7 of the tile
+  auto issue = [&](int nb, int b) {
+#pragma unroll
+    for (int rr = 0; rr < GL_TN / 4; ++rr) {
+      const int r = w * (GL_TN / 4) + rr;
+      const int n = nb + r;
+      if (n < n_end && l < nch)
+        __builtin_amdgcn_global_load_lds(X + (int64_t)n * D + 2 * l, (lds_void*)&sX[b][r * GM_XS], 16, 0, 0);
+    }
+  };
+  __syncthreads();
+  if (tid < GL_TN) sY[0][tid] = (n_begin + tid < n_end) ? y[n_begin + tid] : 0.0;
+  issue(n_begin, 0);
+  int b = 0;
+  for (int nb = n_begin; nb < n_end; nb += GL_TN, b ^= 1) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const int nxt = nb + GL_TN;
+    double yn = 0.0;
+    if (nxt < n_end) {
+      issue(nxt, b ^ 1);
+      if (tid < GL_TN) yn = (nxt + tid < n_end) ? y[nxt + tid] : 0.0;
+    }
+    const double* sx = sX[b];
+    d4 eta[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) eta[rb] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KMAX / 4; ++ks) {
+      if (ks < KS) {
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const double a = sx[(16 * rb + lr) * GM_XS + 4 * ks + lk];
+          eta[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, qreg[ks], eta[rb], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rr = 16 * rb + 4 * i + lk;
+        const double e = eta[rb][i];
+        double r = 0.0;
+        if (nb + rr < n_end) {
+          // t = e^{-|e|}: log1p_exp(e) = max(e,0) + log1p(t), sigmoid(e) from 1 / (1 + t)
+          const double yy = sY[b][rr];
+          const double t = exp(-fabs(e));
+          const double r1 = recip_1to3(1.0 + t);
+          lp += yy * e - ((e > 0.0 ? e : 0.0) + atanh2_series(t * recip_1to3(2.0 + t)));
+          r = (e >= 0.0 ? r1 : t * r1) - yy;
+        }
+        eta[rb][i] = r;
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double a = eta[rb][i];
+        const double* xrow = sx + (16 * rb + 4 * i + lk) * GM_XS + lr;
+#pragma unroll
+        for (int t = 0; t < KMAX / 16; ++t) {
+          if (t < DT) gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xrow[16 * t], gacc[t], 0, 0, 0);
+        }
+      }
+    }
+    if (tid < GL_TN) sY[b ^ 1][tid] = yn;
+  }
+
+#pragma unroll
+  for (int t = 0; t < KMAX / 16; ++t) {
+    if (t < DT) {
+      const int d = 16 * t + lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int pos = c0 + 4 * i + lk;
+        if (pos < n_act && d < D) Wg[((int64_t)chunk * sp.stride + pos) * D + d] = gacc[t][i];
+      }
+    }
+  }
+  lp += __shfl_xor(lp, 16, 64);
+  lp += __shfl_xor(lp, 32, 64);
+  if (lk == 0 && c0 + lr < n_act) Wlp[(int64_t)chunk * sp.stride + c0 + lr] = lp;
+}
+
 __global__ void glm_mfma_reduce(const double* __restrict__ Wg, const double* __restrict__ Wlp, int N,
                                 int total_blocks, int C, int D, double s2inv, const double* __restrict__ Q,
                                 int64_t ldq, const int* __restrict__ idx, const int* __restrict__ n_active,
@@ -246,14 +389,21 @@ int glm_grad_mfma_f64(const GlmArgs& a, const GlmPlan& p, hipStream_t st) {
   if (a.D > GM_KMAX || a.D < 1) return 1;
   double* Wg = a.work;
   double* Wlp = a.work + (size_t)p.total_blocks * GM_TC * a.D;
-#define GM_LAUNCH(K)                                                                                             \
-  hipLaunchKernelGGL(glm_mfma_f64<K>, dim3(p.total_blocks), dim3(256), 0, st, a.X, a.y, a.N, a.D, p.total_blocks, \
-                     a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp)
+  // even D: rows are whole 16-byte chunks, so the double-buffered LDS copy applies
+#define GM_LAUNCH(K)                                                                                                \
+  do {                                                                                                              \
+    if ((a.D & 1) == 0)                                                                                             \
+      hipLaunchKernelGGL(glm_mfma_f64_lds<K>, dim3(p.total_blocks), dim3(256), 0, st, a.X, a.y, a.N, a.D,           \
+                         p.total_blocks, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp);                              \
+    else                                                                                                            \
+      hipLaunchKernelGGL(glm_mfma_f64<K>, dim3(p.total_blocks), dim3(256), 0, st, a.X, a.y, a.N, a.D,               \
+                         p.total_blocks, a.Q, a.ldq, a.C, a.idx, a.n_active, Wg, Wlp);                              \
+  } while (0)
   if (a.ev_begin) (void)hipEventRecord(a.ev_begin, st);
-  // operand registers scale with the D bucket
+  // operand registers scale with the D bucket (a 64 bucket spilled registers:
+  // 33 <= D <= 112 share the 112 instance, whose extra operands are zero)
   if (a.D <= 16) GM_LAUNCH(16);
   else if (a.D <= 32) GM_LAUNCH(32);
-  else if (a.D <= 64) GM_LAUNCH(64);
   else if (a.D <= 112) GM_LAUNCH(112);
   else GM_LAUNCH(128);
 #undef GM_LAUNCH

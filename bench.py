@@ -594,6 +594,9 @@ def main():
         "autotuned_gradient": {"path": ["single-pass", "two-pass", "single-pass bf16-split"][fit.tuned_path]
                                if fit.tuned_path >= 0 else None,
                                "blocks_per_cu": fit.tuned_blocks_per_cu},
+        "autotuned_state_machine": {"threads_per_chain": fit.tuned_threads_per_chain,
+                                    "median_launch_us_wave": fit.tuned_launch_seconds[0] * 1e6,
+                                    "median_launch_us_block": fit.tuned_launch_seconds[1] * 1e6},
         "roofline": roof,
         "batched_learned": learned,
         "cpu_baseline": cpu,

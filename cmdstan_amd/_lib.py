@@ -42,6 +42,8 @@ class nuts_output(C.Structure):
         ("pilot_rhat_plain", C.c_double), ("pilot_rhat_flow", C.c_double),
         ("tuned_path", C.c_int), ("tuned_blocks_per_cu", C.c_int), ("diagnostics", DP),
         ("energy_error", DP), ("init_rejections", C.POINTER(C.c_int)),
+        ("tuned_threads_per_chain", C.c_int), ("tuned_wave_launch_seconds", C.c_double),
+        ("tuned_block_launch_seconds", C.c_double),
     ]
 
 

@@ -398,6 +398,8 @@ struct RunResult {
   int64_t grad_timed = 0;
   long long grad_chains = 0;  // chains evaluated, summed over GLM gradient launches
   int tuned_path = -1, tuned_bpc = 0;
+  int threads_per_chain = 0;                    // lockstep state machine: 64 or 256 (0 fused)
+  double wave_launch_s = 0, block_launch_s = 0;  // timed means when the mode was tuned
   std::vector<double> z_last;  // [C][D] unconstrained q of each chain's last saved row (before any flow map)
   int rows = 0;
   std::vector<double> diag;    // diagnostics: chains back to back, rows x (7 + 3 D)
@@ -767,8 +769,16 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   int maxD = 0;
   for (int c = 0; c < C; ++c) maxD = std::max(maxD, D[c]);
   const char* bpc = std::getenv("CMDSTAN_AMD_BLOCK_PER_CHAIN");
-  const bool block_per_chain =
+  bool block_per_chain =
       !fused && (R.block_mode >= 0 ? R.block_mode == 1 : (bpc ? std::atoi(bpc) != 0 : maxD >= 512));
+  // unless the caller or the environment fixes it, the first check intervals
+  // of the run time one mode each, and the other mode replaces the default by
+  // dimension only when it is more than 20% faster.  The modes order their
+  // reductions differently (draws agree to rounding, not bit for bit), so the
+  // margin keeps near-ties, whose timing could go either way, on the default:
+  // a seed then reproduces its draws
+  const bool tune_threads = !fused && R.block_mode < 0 && !bpc;
+  const bool default_block = block_per_chain;
   DevBuf<double> hlwork;
   HlArgs hla{};
   if (hl_ok && !fused) {
@@ -864,11 +874,18 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
       }
     }
   } ev;
+  EvPairs evs;  // around the state-machine launches of the tuning intervals
+  int tune_left = tune_threads ? 2 : 0;
+  double tune_s[2] = {0, 0};
   DevBuf<long long> tally;
   if (!fused) {
     for (int k = 0; k < kCheck; ++k) {
       HIPC(hipEventCreate(&ev.a[k]));
       HIPC(hipEventCreate(&ev.b[k]));
+      if (tune_threads) {
+        HIPC(hipEventCreate(&evs.a[k]));
+        HIPC(hipEventCreate(&evs.b[k]));
+      }
     }
     tally.alloc(1);
     HIPC(hipMemsetAsync(tally.p, 0, sizeof(long long), st));
@@ -881,12 +898,14 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   int next_report = cfg.refresh > 0 ? cfg.refresh : 0;
   // One check interval of lockstep work: kCheck x (advance, gradient).  With
   // events, each gradient's contraction (or whole gradient) is bracketed.
-  auto enqueue_bundle = [&](bool with_events) {
+  auto enqueue_bundle = [&](bool with_events, bool time_advance = false) {
     // active chains are tallied over the timed launches only (the roofline pairs them with their times)
     long long* tal = with_events ? tally.p : nullptr;
     for (int k = 0; k < kCheck; ++k) {
+      if (time_advance) HIPC(hipEventRecord(evs.a[k], st));
       if (block_per_chain) launch_advance_external_block(cfg.metric, P, B, st);
       else launch_advance_external(cfg.metric, P, B, st);
+      if (time_advance) HIPC(hipEventRecord(evs.b[k], st));
       if (R.flow) {
         // pulled-back gradient: x = T(z) for the active chains, model at x, VJP to z
         launch_compact_active(dneed.p, C, act_idx.p, n_act.p, tal, st);
@@ -962,8 +981,15 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         throw std::runtime_error("sampler did not finish within CMDSTAN_AMD_MAX_SECONDS");
       if (cfg.interrupt && cfg.interrupt(cfg.interrupt_user)) throw std::runtime_error("sampling interrupted");
       bool timed = false;
+      int tuning = -1;  // mode timed in this interval (0 wave, 1 block)
       if (fused) {
         launch_advance_fused(M.kind, cfg.metric, vec32, P, B, md, st);
+      } else if (tune_left > 0) {
+        tuning = tune_left == 2 ? 0 : 1;
+        block_per_chain = tuning == 1;
+        enqueue_bundle(true, true);
+        res.leaf_steps += kCheck;
+        timed = true;
       } else if (use_graph && bundle_no++ % kTimeEvery != 0) {
         if (!graph.x) {
           HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1004,6 +1030,19 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
           res.grad_timed++;
         }
       }
+      if (tuning >= 0) {
+        // median launch of the interval, its first launch left out (it can
+        // carry the one-time cost of the kernel's first use)
+        float t[kCheck - 1];
+        for (int k = 1; k < kCheck; ++k) HIPC(hipEventElapsedTime(&t[k - 1], evs.a[k], evs.b[k]));
+        std::nth_element(t, t + (kCheck - 1) / 2, t + kCheck - 1);
+        tune_s[tuning] = t[(kCheck - 1) / 2] * 1e-3;
+        if (--tune_left == 0) {
+          block_per_chain = default_block ? !(tune_s[0] < 0.8 * tune_s[1]) : (tune_s[1] < 0.8 * tune_s[0]);
+          res.wave_launch_s = tune_s[0];
+          res.block_launch_s = tune_s[1];
+        }
+      }
       if (stopped >= C) break;
     }
   };
@@ -1029,6 +1068,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   auto t3 = std::chrono::steady_clock::now();
   res.warm_s = std::chrono::duration<double>(t1 - t0).count();
   res.samp_s = std::chrono::duration<double>(t3 - t2).count();
+  res.threads_per_chain = fused ? 0 : (block_per_chain ? 256 : 64);
   if (tally.p) {
     HIPC(hipMemcpyAsync(&res.grad_chains, tally.p, sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
@@ -1679,6 +1719,9 @@ int nuts_sample(nuts_model* m, const nuts_config* cfg, const double* init, const
       out->grad_chains_per_launch = r.grad_timed ? (double)r.grad_chains / r.grad_timed : 0.0;
       out->tuned_path = r.tuned_path;
       out->tuned_blocks_per_cu = r.tuned_bpc;
+      out->tuned_threads_per_chain = r.threads_per_chain;
+      out->tuned_wave_launch_seconds = r.wave_launch_s;
+      out->tuned_block_launch_seconds = r.block_launch_s;
       if (cfg->flow) {
         out->warmup_seconds = rep.pre_s;  // plain warmup + pilot, training, flow warmup + pilot
         out->flow_used = rep.used;

@@ -269,7 +269,10 @@ class Model:
                   pilot_ess_per_sec_plain=out.pilot_ess_per_sec_plain,
                   pilot_ess_per_sec_flow=out.pilot_ess_per_sec_flow, pilot_rhat_plain=out.pilot_rhat_plain,
                   pilot_rhat_flow=out.pilot_rhat_flow, tuned_path=out.tuned_path,
-                  tuned_blocks_per_cu=out.tuned_blocks_per_cu, column_names=names,
+                  tuned_blocks_per_cu=out.tuned_blocks_per_cu,
+                  tuned_threads_per_chain=out.tuned_threads_per_chain,
+                  tuned_launch_seconds=(out.tuned_wave_launch_seconds, out.tuned_block_launch_seconds),
+                  column_names=names,
                   dataset_of_chain=ds_of_chain, config=cfg, error=err)
         if rc != 0:
             e = SamplerError(err)

@@ -257,6 +257,14 @@ typedef struct nuts_output {
    * rejected because the log density [0] or its gradient [1] was not finite
    * (the "Rejecting initial value" messages of command_test.cpp:151-250). */
   int* init_rejections;
+  /* lockstep state machine (no counterpart in CmdStan): threads per chain the
+   * run settled on -- 64 (one wave) or 256 (one block), timed on the first
+   * check intervals when neither the caller nor CMDSTAN_AMD_BLOCK_PER_CHAIN
+   * fixes it -- and the median time of one state-machine launch in each
+   * mode over those intervals (0 when not timed; fused mode reports 0 threads) */
+  int tuned_threads_per_chain;
+  double tuned_wave_launch_seconds;
+  double tuned_block_launch_seconds;
 } nuts_output;
 
 /* model_base::transform_inits (the `init=` file path of command_helper.hpp:144-245):

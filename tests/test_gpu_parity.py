@@ -308,6 +308,28 @@ def test_reduced_precision_posterior_matches_fp64(precision, path, monkeypatch):
         assert abs(sa["sd"] - sb["sd"]) < 0.05 * sa["sd"]
 
 
+def test_state_machine_threads_per_chain_tuned(monkeypatch):
+    """Lockstep runs time the wave-per-chain and the block-per-chain state
+    machine on their first two check intervals and report both; the other
+    mode replaces the default by dimension (wave below D = 512) only when it is
+    more than 20% faster.  CMDSTAN_AMD_BLOCK_PER_CHAIN pins the mode and turns
+    the timing off; fused runs report 0 threads."""
+    monkeypatch.delenv("CMDSTAN_AMD_BLOCK_PER_CHAIN", raising=False)
+    gm = ca.Model("logistic_regression", logistic(seed=4, N=3000, D=20))
+    kw = dict(num_chains=64, num_warmup=100, num_samples=50, seed=3, fused=0)
+    fit = gm.sample(**kw)
+    tw, tb = fit.tuned_launch_seconds
+    print(f"state machine launch: wave {tw * 1e6:.1f} us, block {tb * 1e6:.1f} us -> {fit.tuned_threads_per_chain}")
+    assert np.all(fit.status == 0) and tw > 0 and tb > 0
+    assert fit.tuned_threads_per_chain == (256 if tb < 0.8 * tw else 64)
+    monkeypatch.setenv("CMDSTAN_AMD_BLOCK_PER_CHAIN", "1")
+    b = gm.sample(**kw)
+    assert np.all(b.status == 0)
+    assert b.tuned_threads_per_chain == 256 and b.tuned_launch_seconds == (0.0, 0.0)
+    f = ca.Model("gauss", CASES["gauss"]).sample(num_chains=4, num_warmup=50, num_samples=20, seed=1, fused=1)
+    assert f.tuned_threads_per_chain == 0
+
+
 def test_dense_deterministic_and_valid_over_seeds():
     """dense_e adaptation over several seeds: every chain finishes, and a
     repeated run is bit-identical (no read of a stale vector across lanes)."""

@@ -20,7 +20,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 #define FG_BM 64
 #define FG_BN 64
-#define FG_BK 16
+#define FG_BK 64
 enum { FG_EPI_NONE = 0, FG_EPI_BIAS = 1, FG_EPI_BIAS_TANH = 2, FG_EPI_DTANH = 3, FG_EPI_ADD = 4 };
 
 struct GemmDesc {
@@ -53,7 +53,10 @@ __device__ __forceinline__ float fg_epi(const GemmDesc& g, float v, int m, int n
   return v;
 }
 
-// 64x64 tile per block of four waves (2x2), each wave 32x32 = 2x2 MFMA tiles.
+// 64x64 tile per block of four waves (2x2), each wave 32x32 = 2x2 MFMA tiles;
+// K in slabs of 64 (each thread stages 16 elements of A and 16 of B per slab,
+// so a slab's loads are in flight together: the small-K products of the MLP
+// wait on memory once or twice instead of once per 16 of K).
 // f32 16x16x4 operands: lane l gives A[l%16][k=l/16], B[k=l/16][l%16]; result
 // register i of lane l is D[4(l/16)+i][l%16].
 __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
@@ -75,10 +78,12 @@ __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
   const bool a_k = g.sak == 1;
   const bool b_n = g.sbn == 1;
   for (int k0 = kb; k0 < ke; k0 += FG_BK) {
+    float va[16], vb[16];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int mm = a_k ? (t >> 2) : ((t & 15) * 4 + e);
-      const int kk = a_k ? ((t & 3) * 4 + e) : (t >> 4);
+    for (int e = 0; e < 16; ++e) {
+      // k-contiguous A: 16 consecutive k of one row; else 4 rows x 4 slab rows
+      const int mm = a_k ? (t >> 2) : ((t & 15) * 4 + (e & 3));
+      const int kk = a_k ? ((t & 3) * 16 + e) : ((t >> 4) + 16 * (e >> 2));
       const int m = m0 + mm, k = k0 + kk;
       float v = 0.0f;
       if (m < M && k < ke) {
@@ -86,14 +91,23 @@ __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
         const int64_t kc = g.aidx_k ? g.aidx_k[k] : k;
         v = g.A[mr * g.sam + kc * g.sak];
       }
-      As[kk][mm] = v;
+      va[e] = v;
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int nn = b_n ? ((t & 15) * 4 + e) : (t >> 2);
-      const int kk = b_n ? (t >> 4) : ((t & 3) * 4 + e);
+    for (int e = 0; e < 16; ++e) {
+      const int nn = b_n ? ((t & 15) * 4 + (e & 3)) : (t >> 2);
+      const int kk = b_n ? ((t >> 4) + 16 * (e >> 2)) : ((t & 3) * 16 + e);
       const int n = n0 + nn, k = k0 + kk;
-      Bs[kk][nn] = (n < g.N && k < ke) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.0f;
+      vb[e] = (n < g.N && k < ke) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int mm = a_k ? (t >> 2) : ((t & 15) * 4 + (e & 3));
+      const int kk = a_k ? ((t & 3) * 16 + e) : ((t >> 4) + 16 * (e >> 2));
+      As[kk][mm] = va[e];
+      const int nn = b_n ? ((t & 15) * 4 + (e & 3)) : (t >> 2);
+      const int kb2 = b_n ? ((t >> 4) + 16 * (e >> 2)) : ((t & 3) * 16 + e);
+      Bs[kb2][nn] = vb[e];
     }
     __syncthreads();
 #pragma unroll
@@ -852,8 +866,9 @@ void FlowNet::gemm(const float* A, int64_t sam, int64_t sak, const float* B, int
   g.aidx_k = aidx_k; g.aidx_m = aidx_m; g.cidx_n = cidx_n;
   const int tm = (M + FG_BM - 1) / FG_BM, tn = (N + FG_BN - 1) / FG_BN;
   const int kblocks = (K + FG_BK - 1) / FG_BK;
-  // split K until about 512 blocks, keeping >= 4 K-tiles per split
-  int splits = std::max(1, std::min(kblocks / 4, (512 + tm * tn - 1) / (tm * tn)));
+  // split K until about 512 blocks, keeping >= 2 K-slabs (128 of K) per split:
+  // the K = H products of the MLP run unsplit
+  int splits = std::max(1, std::min(kblocks / 2, (512 + tm * tn - 1) / (tm * tn)));
   g.kper = ((kblocks + splits - 1) / splits) * FG_BK;
   splits = (K + g.kper - 1) / g.kper;
   g.splits = splits;

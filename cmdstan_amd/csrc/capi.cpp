@@ -384,6 +384,7 @@ struct RunSpec {
   bool diagnostics = false;                     // record q, p, g of every saved draw
   bool energy_error = false;                    // record each saved row's max |H - H0|
   int block_mode = -1;                          // lockstep state machine: -1 by dimension, 0 wave, 1 block per chain
+  int fused_block = -1;                         // fused kernel, block per chain: -1 once few chains remain, 0 never, 1 always
 };
 
 struct RunResult {
@@ -779,6 +780,18 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   // a seed then reproduces its draws
   const bool tune_threads = !fused && R.block_mode < 0 && !bpc;
   const bool default_block = block_per_chain;
+  // fused kernel: once at most two chains per CU are still running (the tails
+  // of a phase, where the chains left are the largest and deepest), one block
+  // per chain for models whose gradient spreads over a block.  The switch
+  // depends only on how many chains have stopped, so a seed reproduces its
+  // draws; CMDSTAN_AMD_FUSED_BLOCK=0 keeps one wave per chain throughout.
+  const char* fbe = std::getenv("CMDSTAN_AMD_FUSED_BLOCK");
+  const bool fused_block_ok = fused && !M.plugin && model_supports_fused_block(M.kind, cfg.metric) &&
+                              R.fused_block != 0 && !(fbe && std::atoi(fbe) == 0);
+  if (R.fused_block == 1 && !fused_block_ok)
+    throw std::invalid_argument("this model has no block-per-chain fused kernel");
+  const char* fbc = std::getenv("CMDSTAN_AMD_FUSED_BLOCK_CHAINS");  // threshold override (chains still running)
+  const int fused_block_chains = fbc ? std::atoi(fbc) : 2 * num_cus();
   DevBuf<double> hlwork;
   HlArgs hla{};
   if (hl_ok && !fused) {
@@ -983,7 +996,9 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
       bool timed = false;
       int tuning = -1;  // mode timed in this interval (0 wave, 1 block)
       if (fused) {
-        launch_advance_fused(M.kind, cfg.metric, vec32, P, B, md, st);
+        const bool blk = fused_block_ok && (R.fused_block == 1 || C - stopped <= fused_block_chains);
+        if (!(blk && launch_advance_fused_block(M.kind, cfg.metric, vec32, P, B, md, st)))
+          launch_advance_fused(M.kind, cfg.metric, vec32, P, B, md, st);
       } else if (tune_left > 0) {
         tuning = tune_left == 2 ? 0 : 1;
         block_per_chain = tuning == 1;
@@ -1059,6 +1074,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     int done = 0;
     for (auto& s : cur) if (s.phase == PH_DONE || s.phase == PH_FAILED) ++done;
     HIPC(hipMemcpyAsync(dstop.p, &done, sizeof(int), hipMemcpyHostToDevice, st));
+    stopped = done;  // chains still to run in the sampling phase: C - done
   }
   hipLaunchKernelGGL(resume_kernel, dim3((C + 255) / 256), dim3(256), 0, st, dst.p, C);
   P.iter_target = cfg.num_warmup + cfg.num_samples;
@@ -1764,8 +1780,9 @@ int nuts_transition_replay_driver(nuts_model* m, int dataset, int num_chains, in
                                   const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,
                                   int driver, double* out_q, double* info) {
   return guarded([&]() {
-    if (driver < NUTS_DRIVER_FUSED || driver > NUTS_DRIVER_LOCKSTEP_BLOCK)
-      throw std::invalid_argument("driver must be 0 (fused), 1 (lockstep, wave per chain) or 2 (lockstep, block)");
+    if (driver < NUTS_DRIVER_FUSED || driver > NUTS_DRIVER_FUSED_BLOCK)
+      throw std::invalid_argument(
+          "driver must be 0 (fused), 1 (lockstep, wave per chain), 2 (lockstep, block) or 3 (fused, block)");
     RunSpec R;
     nuts_config_default(&R.cfg);
     R.cfg.precision = precision;
@@ -1773,8 +1790,10 @@ int nuts_transition_replay_driver(nuts_model* m, int dataset, int num_chains, in
     R.cfg.num_warmup = 0; R.cfg.num_samples = 1; R.cfg.adapt_engaged = 0;
     R.cfg.metric = metric; R.cfg.stepsize = stepsize; R.cfg.stepsize_jitter = jitter;
     R.cfg.max_depth = max_depth; R.cfg.num_chains = num_chains;
-    R.cfg.fused = driver == NUTS_DRIVER_FUSED ? 1 : 0;
-    R.block_mode = driver == NUTS_DRIVER_FUSED ? -1 : (driver == NUTS_DRIVER_LOCKSTEP_BLOCK ? 1 : 0);
+    const bool fused_drv = driver == NUTS_DRIVER_FUSED || driver == NUTS_DRIVER_FUSED_BLOCK;
+    R.cfg.fused = fused_drv ? 1 : 0;
+    R.block_mode = fused_drv ? -1 : (driver == NUTS_DRIVER_LOCKSTEP_BLOCK ? 1 : 0);
+    R.fused_block = driver == NUTS_DRIVER_FUSED_BLOCK ? 1 : (driver == NUTS_DRIVER_FUSED ? 0 : -1);
     R.init = q0; R.inv_metric = inv_metric;
     R.inj_u = u; R.n_u = n_u; R.inj_z = z; R.n_z = n_z;
     R.first_dataset = dataset; R.num_datasets = 1;

@@ -36,6 +36,11 @@ void launch_dense_factor(const NutsBuffers& B, int* failed, hipStream_t st);
 // dot products and tree weights stay FP64
 void launch_advance_fused(int kind, int metric, bool vec32, const NutsParams& P, const NutsBuffers& B,
                           const ModelDataDev& md, hipStream_t st);
+// the same with one 256-thread block per chain (false: the model has no
+// block-wide gradient, or dense_e)
+bool launch_advance_fused_block(int kind, int metric, bool vec32, const NutsParams& P, const NutsBuffers& B,
+                                const ModelDataDev& md, hipStream_t st);
+bool model_supports_fused_block(int kind, int metric);
 bool model_supports_vec32(int kind);
 
 // Per-chain model gradient: V[c], G row c at Q row c for chains with need[c]

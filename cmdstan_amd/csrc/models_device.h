@@ -21,6 +21,7 @@ struct ModelData {
 // block = [N, sum(y)], q = logit(theta)
 struct BernoulliFn {
   static constexpr bool fused = true;
+  static constexpr bool block_grad = false;
   static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
   template <class T>
@@ -43,6 +44,7 @@ struct BernoulliFn {
 // q ~ multi_normal_prec(mu, P); block = [D, mu[D], P[D*D] row-major]
 struct GaussFn {
   static constexpr bool fused = true;
+  static constexpr bool block_grad = false;
   static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
   template <class T>
@@ -72,6 +74,7 @@ struct GaussFn {
 // block = [J, mu_s, tau_s, y[J], sigma[J]]
 struct EightSchoolsFn {
   static constexpr bool fused = true;
+  static constexpr bool block_grad = false;
   static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
   template <class T>
@@ -114,10 +117,13 @@ struct EightSchoolsFn {
 // block = [J, mu_s, tau_s, y[J], sigma[J]]; q = (mu, log tau, eta); outputs mu, tau, eta, theta
 struct HierNormalNCFn {
   static constexpr bool fused = true;
-  static constexpr bool vec32 = true;  // FP32 vector storage supported
+  static constexpr bool vec32 = true;       // FP32 vector storage supported
+  static constexpr bool block_grad = true;  // gradient over a block per chain (group)
   ModelData md;
-  template <class T>
-  __device__ double operator()(int c, const T* __restrict__ q, T* __restrict__ gV, int, int lane) const {
+  // the gradient with TPC threads per chain (thread tid); sum reduces a
+  // per-thread partial over the chain's threads in a fixed order
+  template <int TPC, class T, class Sum>
+  __device__ double group(int c, const T* __restrict__ q, T* __restrict__ gV, int tid, const Sum& sum) const {
     const double* d = md.data + md.off[c];
     const int J = (int)d[0];
     const double mu_s = d[1], tau_s = d[2];
@@ -126,7 +132,7 @@ struct HierNormalNCFn {
     const double mu = q[0], v = q[1], tau = exp(v);
     double acc = 0, gmu = 0, gtau = 0;
 #pragma unroll 4
-    for (int j = lane; j < J; j += NUTS_WAVE) {
+    for (int j = tid; j < J; j += TPC) {
       const double eta = q[2 + j];
       const double th = mu + tau * eta;
       const double e = (y[j] - th) / sg[j];
@@ -136,14 +142,18 @@ struct HierNormalNCFn {
       gtau += r * eta;
       gV[2 + j] = eta - r * tau;
     }
-    acc = wave_sum(acc); gmu = wave_sum(gmu); gtau = wave_sum(gtau);
+    acc = sum(acc); gmu = sum(gmu); gtau = sum(gtau);
     double lp = -0.5 * mu * mu / (mu_s * mu_s) - log(1.0 + tau * tau / (tau_s * tau_s));
     lp += md.jac * v;
     lp += acc;
     gmu += -mu / (mu_s * mu_s);
     gtau += -2.0 * tau / (tau_s * tau_s + tau * tau);
-    if (lane == 0) { gV[0] = -gmu; gV[1] = -(md.jac + tau * gtau); }
+    if (tid == 0) { gV[0] = -gmu; gV[1] = -(md.jac + tau * gtau); }
     return -lp;
+  }
+  template <class T>
+  __device__ double operator()(int c, const T* __restrict__ q, T* __restrict__ gV, int, int lane) const {
+    return group<NUTS_WAVE>(c, q, gV, lane, [](double x) { return wave_sum(x); });
   }
   __device__ int nout(int c) const { return 2 + 2 * (int)md.data[md.off[c]]; }
   __device__ void write_array(int c, const double* q, double* out) const {
@@ -160,6 +170,7 @@ struct HierNormalNCFn {
 // block = [N, K, G, mu_s, gstart[G+1], X[N*K], y[N]]; q = (mu[K], log tau[K], beta[G][K])
 struct HierLogisticFn {
   static constexpr bool fused = true;
+  static constexpr bool block_grad = false;
   static constexpr bool vec32 = true;  // FP32 vector storage supported
   ModelData md;
   template <class T>
@@ -231,6 +242,7 @@ struct HierLogisticFn {
 // block = [N, D, 1/s^2, X[N*D], y[N]]
 struct LogisticFn {
   static constexpr bool fused = true;
+  static constexpr bool block_grad = false;
   static constexpr bool vec32 = false;
   ModelData md;
   __device__ double operator()(int c, const double* __restrict__ q, double* __restrict__ gV, int D, int lane) const {

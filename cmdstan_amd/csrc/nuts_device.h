@@ -1249,14 +1249,19 @@ struct NutsWave {
       if (s.phase >= PH_PAUSED) return;
       const bool req = dispatch();
       if (!req) continue;
-      if (!GradFn::fused) { if (tid == 0) B.need[c] = 1; converge(); return; }
-      gsync();  // the model reads q written lane-strided
-      const double V = grad(c, q, g, D, lane);  // fused kernels run one wave per chain
-      v_new = V;
-      have_v = true;
-      if (tid == 0) B.V[c] = V; converge();  // for the next launch only
-      gsync();  // models may write g elements that other lanes read
-      if (--budget == 0) return;
+      if constexpr (!GradFn::fused) {
+        if (tid == 0) B.need[c] = 1; converge(); return;
+      } else {
+        gsync();  // the model reads q written lane-strided
+        double V;
+        if constexpr (TPC == NUTS_WAVE) V = grad(c, q, g, D, lane);
+        else V = grad.template group<TPC>(c, q, g, tid, [](double x) { return gsum(x); });  // a block per chain
+        v_new = V;
+        have_v = true;
+        if (tid == 0) B.V[c] = V; converge();  // for the next launch only
+        gsync();  // models may write g elements that other lanes read
+        if (--budget == 0) return;
+      }
     }
   }
 

@@ -24,6 +24,13 @@ __global__ void __launch_bounds__(256) advance_fused_kernel(NutsParams P, NutsBu
   nuts_advance_body<METRIC, F, NUTS_WAVE, VT>(P, B, f);
 }
 
+// one 256-thread block per chain with the model's gradient spread over the
+// block (models with F::block_grad): for few, large chains
+template <int METRIC, class F, class VT>
+__global__ void __launch_bounds__(256) advance_fused_block_kernel(NutsParams P, NutsBuffers B, F f) {
+  nuts_advance_body<METRIC, F, 256, VT>(P, B, f);
+}
+
 template <class F>
 __global__ void __launch_bounds__(256) model_grad_kernel(F f, int C, const int* D, const int64_t* off, const double* Q,
                                                          double* G, double* V, int* need) {
@@ -103,6 +110,29 @@ static void fused_for_metric(int metric, bool vec32, const NutsParams& P, const 
   else hipLaunchKernelGGL((advance_fused_kernel<METRIC_DIAG, F, double>), grid, blk, 0, st, P, B, f);
 }
 
+template <class F>
+static bool fused_block_for_metric(int metric, bool vec32, const NutsParams& P, const NutsBuffers& B, const F& f,
+                                   hipStream_t st) {
+  if constexpr (!F::block_grad) {
+    return false;
+  } else {
+    if (metric == METRIC_DENSE) return false;
+    const dim3 grid(B.C), blk(256);
+    if constexpr (F::vec32) {
+      if (vec32) {
+        if (metric == METRIC_UNIT)
+          hipLaunchKernelGGL((advance_fused_block_kernel<METRIC_UNIT, F, float>), grid, blk, 0, st, P, B, f);
+        else hipLaunchKernelGGL((advance_fused_block_kernel<METRIC_DIAG, F, float>), grid, blk, 0, st, P, B, f);
+        return true;
+      }
+    }
+    if (metric == METRIC_UNIT)
+      hipLaunchKernelGGL((advance_fused_block_kernel<METRIC_UNIT, F, double>), grid, blk, 0, st, P, B, f);
+    else hipLaunchKernelGGL((advance_fused_block_kernel<METRIC_DIAG, F, double>), grid, blk, 0, st, P, B, f);
+    return true;
+  }
+}
+
 #define NUTS_MODEL_SWITCH(kind, md, CALL)                       \
   switch (kind) {                                               \
     case MK_BERNOULLI: { BernoulliFn f{{md.data, md.off, md.jac}}; CALL; break; }    \
@@ -117,6 +147,20 @@ static void fused_for_metric(int metric, bool vec32, const NutsParams& P, const 
 void launch_advance_fused(int kind, int metric, bool vec32, const NutsParams& P, const NutsBuffers& B,
                           const ModelDataDev& md, hipStream_t st) {
   NUTS_MODEL_SWITCH(kind, md, fused_for_metric(metric, vec32, P, B, f, st));
+}
+
+bool launch_advance_fused_block(int kind, int metric, bool vec32, const NutsParams& P, const NutsBuffers& B,
+                                const ModelDataDev& md, hipStream_t st) {
+  bool ok = false;
+  NUTS_MODEL_SWITCH(kind, md, ok = fused_block_for_metric(metric, vec32, P, B, f, st));
+  return ok;
+}
+
+bool model_supports_fused_block(int kind, int metric) {
+  bool r = false;
+  ModelDataDev md{nullptr, nullptr, 1.0};
+  NUTS_MODEL_SWITCH(kind, md, r = decltype(f)::block_grad);
+  return r && metric != METRIC_DENSE;
 }
 
 bool model_supports_vec32(int kind) {

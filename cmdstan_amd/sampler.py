@@ -284,7 +284,9 @@ class Model:
                           max_depth=10, dataset=0, precision=0, driver="fused"):
         """One transition per chain with injected draws.  driver: "fused" (the
         per-chain kernel), "lockstep" (a wave per chain and the model's batched
-        gradient kernel) or "lockstep_block" (a 256-thread block per chain)."""
+        gradient kernel), "lockstep_block" (a 256-thread block per chain) or
+        "fused_block" (the per-chain kernel with a block per chain, for models
+        whose gradient spreads over a block)."""
         q0 = _f64(np.atleast_2d(q0))
         nch, D = q0.shape
         u = _f64(np.atleast_2d(uniforms))
@@ -299,7 +301,7 @@ class Model:
         out_q = np.zeros((nch, D))
         info = np.zeros((nch, 9))
         p = {"fp64": 0, "fp32": 1, "bf16": 2}.get(precision, precision)
-        drv = {"fused": 0, "lockstep": 1, "lockstep_block": 2}.get(driver, driver)
+        drv = {"fused": 0, "lockstep": 1, "lockstep_block": 2, "fused_block": 3}.get(driver, driver)
         rc = lib().nuts_transition_replay_driver(self.h, dataset, nch, m, stepsize, jitter, max_depth, _dp(im),
                                                  _dp(q0), _dp(u), u.shape[1], _dp(z), z.shape[1], p, drv,
                                                  _dp(out_q), _dp(info))

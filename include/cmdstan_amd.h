@@ -303,10 +303,14 @@ int nuts_transition_replay_precision(nuts_model* m, int dataset, int num_chains,
  * with one wavefront per chain and the model's batched gradient kernel (the
  * FP64 matrix-core GEMM for logistic regression, the group kernel for the
  * hierarchical logistic model), 2 the lockstep kernels with one 256-thread
- * block per chain.  No counterpart in the reference. */
+ * block per chain, 3 the fused kernel with one 256-thread block per chain
+ * (models whose gradient spreads over a block; unit/diag).  Sampling uses
+ * driver 3 by itself for the last chains of a phase.  No counterpart in the
+ * reference. */
 #define NUTS_DRIVER_FUSED 0
 #define NUTS_DRIVER_LOCKSTEP_WAVE 1
 #define NUTS_DRIVER_LOCKSTEP_BLOCK 2
+#define NUTS_DRIVER_FUSED_BLOCK 3
 int nuts_transition_replay_driver(nuts_model* m, int dataset, int num_chains, int metric, double stepsize,
                                   double jitter, int max_depth, const double* inv_metric, const double* q0,
                                   const double* u, int64_t n_u, const double* z, int64_t n_z, int precision,

@@ -101,3 +101,21 @@ def test_lockstep_block_replay_logistic_dense_d1000():
     data = logistic(seed=8, N=1500, D=1000)
     _replay_and_compare("logistic_regression", data, "lockstep_block", "dense_e", [0.003, 0.01], nch=4,
                         max_depth=6, q_scale=0.02, q_tol=1e-8)
+
+
+@pytest.mark.parametrize("metric", ["unit_e", "diag_e"])
+def test_fused_block_replay_hier_normal_nc(metric):
+    """The fused kernel with one 256-thread block per chain (the driver the
+    sampler takes for the last chains of a phase) on the non-centred
+    hierarchical normal model, J = 700 (D = 702: each thread owns a few
+    elements, the block-wide sums run through LDS), against the oracle."""
+    data = hier_normal(seed=31, J=700)
+    _replay_and_compare("hier_normal_nc", data, "fused_block", metric, (0.05, 0.12, 0.25), nch=12, max_depth=9,
+                        q_scale=0.3)
+
+
+def test_fused_block_driver_is_refused_for_other_models():
+    gm = ca.Model("gauss", gauss(seed=2, D=4))
+    q0 = np.zeros((2, 4))
+    with pytest.raises(ca.SamplerError, match="block-per-chain"):
+        gm.transition_replay(q0, np.full((2, 64), 0.5), np.zeros((2, 4)), 0.1, driver="fused_block")

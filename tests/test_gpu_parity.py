@@ -687,3 +687,22 @@ def test_diagnostic_rows_rejected_with_flow():
     gm = ca.Model("gauss", CASES["gauss"])
     with pytest.raises(ca.SamplerError):
         gm.sample(num_chains=2, num_warmup=20, num_samples=10, flow="force", diagnostics=True)
+
+
+def test_fused_block_sampling_matches_wave(monkeypatch):
+    """The non-centred hierarchical model sampled by the fused kernel with one
+    block per chain (what a run takes once at most two chains per CU remain,
+    here from the start) and with one wave per chain
+    (CMDSTAN_AMD_FUSED_BLOCK=0): every chain finishes and the posterior means
+    of mu, tau and two thetas agree within 4 combined MCSE."""
+    from tests.model_cases import hier_normal
+    m = ca.Model("hier_normal_nc", hier_normal(seed=8, J=300))
+    kw = dict(num_chains=64, num_warmup=300, num_samples=300, fused=1)
+    monkeypatch.delenv("CMDSTAN_AMD_FUSED_BLOCK", raising=False)
+    a = m.sample(seed=11, **kw)
+    monkeypatch.setenv("CMDSTAN_AMD_FUSED_BLOCK", "0")
+    b = m.sample(seed=12, **kw)
+    assert np.all(a.status == 0) and np.all(b.status == 0)
+    for name in ("mu", "tau", "theta.1", "theta.150"):
+        sa, sb = E.summary(a.param(name)), E.summary(b.param(name))
+        assert abs(sa["mean"] - sb["mean"]) < 4 * np.hypot(sa["mcse"], sb["mcse"]), (name, sa, sb)

@@ -1065,6 +1065,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   HIPC(hipStreamSynchronize(st));
   auto t0 = std::chrono::steady_clock::now();
   run_phase();
+  if (R.flow) R.flow->check_barriers();
   auto t1 = std::chrono::steady_clock::now();
   // resume paused chains for the sampling phase
   {
@@ -1081,6 +1082,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
   HIPC(hipStreamSynchronize(st));
   auto t2 = std::chrono::steady_clock::now();
   run_phase();
+  if (R.flow) R.flow->check_barriers();
   auto t3 = std::chrono::steady_clock::now();
   res.warm_s = std::chrono::duration<double>(t1 - t0).count();
   res.samp_s = std::chrono::duration<double>(t3 - t2).count();
@@ -1975,6 +1977,7 @@ int nuts_flow_forward(nuts_flow* f, const double* z, int n, double* x, double* l
     HIPC(hipMemcpyAsync(x, dx.p, (size_t)n * D * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(logdet, dl.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
+    f->net->check_barriers();
     return NUTS_OK;
   });
 }
@@ -2007,6 +2010,7 @@ int nuts_flow_pullback_grad(nuts_flow* f, const double* z, const double* gx, int
     f->net->pullback(dgx.p, dvx.p, dl.p, nullptr, nullptr, n, dgz.p, D, dv.p);
     HIPC(hipMemcpyAsync(gz, dgz.p, (size_t)n * D * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
+    f->net->check_barriers();
     return NUTS_OK;
   });
 }

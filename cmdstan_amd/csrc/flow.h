@@ -17,6 +17,7 @@
 namespace nutsgpu {
 
 struct ModelDataDev;
+struct FpArgs;  // arguments of the persistent sampling kernels (flow.hip)
 
 struct FlowSpec {
   int D = 0, L = 0, H = 0;
@@ -60,6 +61,10 @@ class FlowNet {
   // In place on n_rows sampler rows (stride ld, z in columns 7..7+D): z -> x = T(z),
   // column 0 (lp of z) -> lp(z) - log|det dT/dz|.
   void map_rows(double* rows, int64_t ld, int n_rows);
+  // Throws if a grid barrier of the persistent sampling kernels timed out
+  // since the last check (their results are then not to be used); synchronizes
+  // the stream when those kernels have run.
+  void check_barriers();
   // z = T^{-1}(x) for n_rows rows of X (stride D), n_rows <= max_rows.
   void inverse(const double* X, int n_rows, double* Z, double* logdet);
 
@@ -81,6 +86,8 @@ class FlowNet {
             int64_t ldc, int M, int N, int K, const int* Mdev, int epi, const float* bias, const float* aux,
             int64_t ldaux, const int* aidx_k = nullptr, const int* aidx_m = nullptr, const int* cidx_n = nullptr);
   bool fused_ok(int n_rows) const;  // the fused per-layer kernels fit (H, partial workspace)
+  bool persistent_ok(int n_rows) const;  // the persistent forward / pullback kernels apply
+  void persistent_args(FpArgs& a, const int* idx, const int* n_dev, int n_rows) const;
   void inverse_pass(int rows);     // U_[L] -> U_[0], keeping the training tape
   void row_losses(int rows);       // rowbuf_ = per-row loss after inverse_pass
   void backward_params(int rows);  // parameter gradient of the mean loss into grad_
@@ -105,6 +112,11 @@ class FlowNet {
   int64_t* pick_ = nullptr;   // R minibatch rows
   int* db_dev_ = nullptr;     // L
   double* ld_part_ = nullptr; // L x ceil(Dmax_b / 64) x R: per-tile row sums of s (fused forward)
+  unsigned* bar_ = nullptr;   // grid-barrier counters of the persistent kernels (arrivals, departures)
+  int* bar_err_ = nullptr;    // set on the device when a grid barrier times out
+  int n_cu_ = 0;              // blocks of a persistent launch (one per CU)
+  uint64_t spin_ticks_ = 0;   // wall-clock ticks a grid barrier may wait (2 s)
+  bool ran_persistent_ = false;
   int64_t adam_t_ = 0;
 };
 

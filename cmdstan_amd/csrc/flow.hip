@@ -55,8 +55,9 @@ __device__ __forceinline__ float fg_epi(const GemmDesc& g, float v, int m, int n
 
 // 64x64 tile per block of four waves (2x2), each wave 32x32 = 2x2 MFMA tiles;
 // K in slabs of 64 (each thread stages 16 elements of A and 16 of B per slab,
-// so a slab's loads are in flight together: the small-K products of the MLP
-// wait on memory once or twice instead of once per 16 of K).
+// so a slab's loads are in flight together, and the next slab's loads are
+// issued before the current slab's products: a tile waits on memory about
+// once, however long its K range).
 // f32 16x16x4 operands: lane l gives A[l%16][k=l/16], B[k=l/16][l%16]; result
 // register i of lane l is D[4(l/16)+i][l%16].
 __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
@@ -77,8 +78,8 @@ __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
     for (int j = 0; j < 2; ++j) acc[i][j] = f4{0, 0, 0, 0};
   const bool a_k = g.sak == 1;
   const bool b_n = g.sbn == 1;
-  for (int k0 = kb; k0 < ke; k0 += FG_BK) {
-    float va[16], vb[16];
+  float va[16], vb[16];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       // k-contiguous A: 16 consecutive k of one row; else 4 rows x 4 slab rows
@@ -100,6 +101,10 @@ __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
       const int n = n0 + nn, k = k0 + kk;
       vb[e] = (n < g.N && k < ke) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.0f;
     }
+  };
+  // the next slab's loads are issued before the current slab's products
+  if (kb < ke) fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += FG_BK) {
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int mm = a_k ? (t >> 2) : ((t & 15) * 4 + (e & 3));
@@ -110,6 +115,7 @@ __global__ void __launch_bounds__(256) flow_gemm_kernel(GemmDesc g) {
       Bs[kb2][nn] = vb[e];
     }
     __syncthreads();
+    if (k0 + FG_BK < ke) fetch(k0 + FG_BK);
 #pragma unroll
     for (int ks = 0; ks < FG_BK / 4; ++ks) {
       const int kr = ks * 4 + (l >> 4);
@@ -162,35 +168,52 @@ __device__ __forceinline__ int rows_of(const int* n_dev, int n) { return n_dev ?
 //             GP2 = sum (1 - h2^2), GP1 = (GP2 W2^T)(1 - h1^2)
 //             GC[:, ia] += GP1 W1^T
 // Tiles: 64 rows x 64 columns per block of four waves (2 x 2 of 32 x 32), K in
-// steps of 16 staged through LDS, v_mfma_f32_16x16x4f32.  Every sum has a
+// slabs of 64 staged through LDS, v_mfma_f32_16x16x4f32.  Every sum has a
 // fixed order (no atomics): results are bit-reproducible.
 #define FF_LD 68       // LDS row stride of the staged operand slabs (floats)
 #define FF_HMAX 256    // widest MLP the fused kernels hold in LDS (64 x H tile)
 
-// acc += A(m, k) B(k, n) over k in [kb, ke) for the block's 64 x 64 tile.
-// A_KC: consecutive k of A are adjacent in memory (a thread loads 4 of one
-// row); B_NC: consecutive n of B are adjacent (a thread loads 4 of one k).
+// acc += A(m, k) B(k, n) over k in [kb, ke) for the block's 64 x 64 tile, K
+// in slabs of 64: each thread holds 16 elements of A and 16 of B of a slab in
+// registers, and the next slab's loads are issued before the current slab's
+// products, so a tile waits on memory about once however long K is.
+// A_KC: consecutive k of A are adjacent in memory (a thread loads 16 of one
+// row); B_NC: consecutive n of B are adjacent (a thread loads 4 of one k, in
+// 4 slab rows).  Products in k order, four at a time (v_mfma_f32_16x16x4f32).
+#define FF_BK 64
 template <bool A_KC, bool B_NC, class AF, class BF>
 __device__ __forceinline__ void ff_mma(const AF& af, const BF& bf, int kb, int ke, f4 (&acc)[2][2],
                                        float (*As)[FF_LD], float (*Bs)[FF_LD]) {
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  for (int k0 = kb; k0 < ke; k0 += 16) {
+  auto a_m = [&](int e) { return A_KC ? (t >> 2) : ((t & 15) * 4 + (e & 3)); };
+  auto a_k = [&](int e) { return A_KC ? ((t & 3) * 16 + e) : ((t >> 4) + 16 * (e >> 2)); };
+  auto b_n = [&](int e) { return B_NC ? ((t & 15) * 4 + (e & 3)) : (t >> 2); };
+  auto b_k = [&](int e) { return B_NC ? ((t >> 4) + 16 * (e >> 2)) : ((t & 3) * 16 + e); };
+  float ra[16], rb[16];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int mm = A_KC ? (t >> 2) : ((t & 15) * 4 + e);
-      const int kk = A_KC ? ((t & 3) * 4 + e) : (t >> 4);
-      As[kk][mm] = (k0 + kk < ke) ? af(mm, k0 + kk) : 0.0f;
+    for (int e = 0; e < 16; ++e) {
+      const int k = k0 + a_k(e);
+      ra[e] = k < ke ? af(a_m(e), k) : 0.0f;
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int nn = B_NC ? ((t & 15) * 4 + e) : (t >> 2);
-      const int kk = B_NC ? (t >> 4) : ((t & 3) * 4 + e);
-      Bs[kk][nn] = (k0 + kk < ke) ? bf(k0 + kk, nn) : 0.0f;
+    for (int e = 0; e < 16; ++e) {
+      const int k = k0 + b_k(e);
+      rb[e] = k < ke ? bf(k, b_n(e)) : 0.0f;
+    }
+  };
+  if (kb < ke) fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += FF_BK) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      As[a_k(e)][a_m(e)] = ra[e];
+      Bs[b_k(e)][b_n(e)] = rb[e];
     }
     __syncthreads();
+    if (k0 + FF_BK < ke) fetch(k0 + FF_BK);
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
+    for (int ks = 0; ks < FF_BK / 4; ++ks) {  // slab rows past ke hold zeros
       const int kr = ks * 4 + (l >> 4);
       const float a0 = As[kr][wm + (l & 15)], a1 = As[kr][wm + 16 + (l & 15)];
       const float b0 = Bs[kr][wn + (l & 15)], b1 = Bs[kr][wn + 16 + (l & 15)];
@@ -225,41 +248,58 @@ __device__ __forceinline__ void ff_store(const f4 (&acc)[2][2], const F& f) {
       for (int r = 0; r < 4; ++r) f(wm + 16 * i + 4 * (l >> 4) + r, wn + 16 * j + (l & 15), acc[i][j][r]);
 }
 
-// forward 1: h1 pre-activations U[:, ia] W1 over the conditioner columns
-// [z kper, (z+1) kper); with one split the epilogue writes h1 = tanh(. + b1)
-__global__ void __launch_bounds__(256) ff_h1(const float* __restrict__ U, int D, const int* __restrict__ ia, int Da,
-                                             const float* __restrict__ W1, const float* __restrict__ b1, int H,
-                                             const int* __restrict__ n_dev, int n, int kper, int splits,
-                                             float* __restrict__ h1, float* __restrict__ part) {
-  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+// LDS of one fused tile: the two staged operand slabs, then the tile's own
+// buffer (at most 64 x (FF_HMAX + 1) floats).
+#define FF_SLABS (2 * FF_BK * FF_LD)
+#define FF_POOL (FF_SLABS + 64 * (FF_HMAX + 1))
+typedef float (*ff_slab)[FF_LD];
+__device__ __forceinline__ ff_slab ff_as(float* pool) { return reinterpret_cast<ff_slab>(pool); }
+__device__ __forceinline__ ff_slab ff_bs(float* pool) { return reinterpret_cast<ff_slab>(pool + FF_BK * FF_LD); }
+
+// forward 1, tile (bx, by, bz): h1 pre-activations U[:, ia] W1 over the
+// conditioner columns [bz kper, (bz+1) kper); with one split the epilogue
+// writes h1 = tanh(. + b1)
+__device__ __forceinline__ void ff_h1_tile(int bx, int by, int bz, float* pool, const float* __restrict__ U, int D,
+                                           const int* __restrict__ ia, int Da, const float* __restrict__ W1,
+                                           const float* __restrict__ b1, int H, const int* __restrict__ n_dev, int n,
+                                           int kper, int splits, float* __restrict__ h1, float* __restrict__ part) {
   const int M = rows_of(n_dev, n);
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int m0 = by * 64, n0 = bx * 64;
   if (m0 >= M) return;
-  const int kb = blockIdx.z * kper, ke = min(Da, kb + kper);
+  const int kb = bz * kper, ke = min(Da, kb + kper);
   f4 acc[2][2];
   ff_zero(acc);
   auto af = [&](int mm, int k) { const int m = m0 + mm; return m < M ? U[(int64_t)m * D + ia[k]] : 0.0f; };
   auto bf = [&](int k, int nn) { const int c = n0 + nn; return c < H ? W1[(int64_t)k * H + c] : 0.0f; };
-  ff_mma<true, true>(af, bf, kb, ke, acc, As, Bs);
+  ff_mma<true, true>(af, bf, kb, ke, acc, ff_as(pool), ff_bs(pool));
   ff_store(acc, [&](int mm, int nn, float v) {
     const int m = m0 + mm, c = n0 + nn;
     if (m >= M || c >= H) return;
     if (splits == 1) h1[(int64_t)m * H + c] = tanhf(v + b1[c]);
-    else part[((int64_t)blockIdx.z * n + m) * H + c] = v;
+    else part[((int64_t)bz * n + m) * H + c] = v;
   });
 }
 
-// forward 2: h1 = tanh(sum of the partials + b1) for the block's rows (all H
-// columns, kept in LDS; column tile 0 stores it), h2 = tanh(h1 W2 + b2)
-__global__ void __launch_bounds__(256) ff_h2(const float* __restrict__ part, int splits, const float* __restrict__ b1,
-                                             float* __restrict__ h1, const float* __restrict__ W2,
-                                             const float* __restrict__ b2, float* __restrict__ h2, int H,
-                                             const int* __restrict__ n_dev, int n) {
-  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
-  __shared__ float sh[64][FF_HMAX + 1];
+__global__ void __launch_bounds__(256) ff_h1(const float* __restrict__ U, int D, const int* __restrict__ ia, int Da,
+                                             const float* __restrict__ W1, const float* __restrict__ b1, int H,
+                                             const int* __restrict__ n_dev, int n, int kper, int splits,
+                                             float* __restrict__ h1, float* __restrict__ part) {
+  __shared__ float pool[FF_SLABS];
+  ff_h1_tile(blockIdx.x, blockIdx.y, blockIdx.z, pool, U, D, ia, Da, W1, b1, H, n_dev, n, kper, splits, h1, part);
+}
+
+// forward 2, tile (bx, by): h1 = tanh(sum of the partials + b1) for the
+// tile's rows (all H columns, kept in LDS; column tile 0 stores it; with one
+// split h1 is read as stored), h2 = tanh(h1 W2 + b2)
+__device__ __forceinline__ void ff_h2_tile(int bx, int by, float* pool, const float* __restrict__ part, int splits,
+                                           const float* __restrict__ b1, float* __restrict__ h1,
+                                           const float* __restrict__ W2, const float* __restrict__ b2,
+                                           float* __restrict__ h2, int H, const int* __restrict__ n_dev, int n) {
+  float* sh = pool + FF_SLABS;  // [64][FF_HMAX + 1]
   const int M = rows_of(n_dev, n);
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int m0 = by * 64, n0 = bx * 64;
   if (m0 >= M) return;
+#pragma unroll 8
   for (int i = threadIdx.x; i < 64 * H; i += 256) {
     const int mm = i / H, c = i % H, m = m0 + mm;
     float v = 0.0f;
@@ -270,46 +310,64 @@ __global__ void __launch_bounds__(256) ff_h2(const float* __restrict__ part, int
         float s = 0.0f;
         for (int z = 0; z < splits; ++z) s += part[((int64_t)z * n + m) * H + c];
         v = tanhf(s + b1[c]);
-        if (blockIdx.x == 0) h1[(int64_t)m * H + c] = v;
+        if (bx == 0) h1[(int64_t)m * H + c] = v;
       }
     }
-    sh[mm][c] = v;
+    sh[mm * (FF_HMAX + 1) + c] = v;
   }
   __syncthreads();
   f4 acc[2][2];
   ff_zero(acc);
-  auto af = [&](int mm, int k) { return sh[mm][k]; };
+  auto af = [&](int mm, int k) { return sh[mm * (FF_HMAX + 1) + k]; };
   auto bf = [&](int k, int nn) { const int c = n0 + nn; return c < H ? W2[(int64_t)k * H + c] : 0.0f; };
-  ff_mma<true, true>(af, bf, 0, H, acc, As, Bs);
+  ff_mma<true, true>(af, bf, 0, H, acc, ff_as(pool), ff_bs(pool));
   ff_store(acc, [&](int mm, int nn, float v) {
     const int m = m0 + mm, c = n0 + nn;
     if (m < M && c < H) h2[(int64_t)m * H + c] = tanhf(v + b2[c]);
   });
+  __syncthreads();  // sh is reused by the next tile of a persistent block
 }
 
-// forward 3: [s_raw, t] = h2 W3 + b3 for the transformed columns
-// [j0, j0 + 64) (s at column j, t at Db + j), then the coupling
-// u_b e^s + t with s = 3 tanh(s_raw / 3), S = s, the row sums of s over the
-// tile (ld_part, FP64) and this block's share of the conditioner columns
-// copied through
-__global__ void __launch_bounds__(256) ff_couple(const float* __restrict__ h2, int H, const float* __restrict__ W3,
-                                                 const float* __restrict__ b3, const int* __restrict__ ia, int Da,
-                                                 const int* __restrict__ ib, int Db, int D,
-                                                 const float* __restrict__ U, float* __restrict__ Un,
-                                                 float* __restrict__ S, int64_t lds, double* __restrict__ ld_part,
-                                                 const int* __restrict__ n_dev, int n) {
-  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
-  __shared__ float ss[64][65];
+__global__ void __launch_bounds__(256) ff_h2(const float* __restrict__ part, int splits, const float* __restrict__ b1,
+                                             float* __restrict__ h1, const float* __restrict__ W2,
+                                             const float* __restrict__ b2, float* __restrict__ h2, int H,
+                                             const int* __restrict__ n_dev, int n) {
+  __shared__ float pool[FF_POOL];
+  ff_h2_tile(blockIdx.x, blockIdx.y, pool, part, splits, b1, h1, W2, b2, h2, H, n_dev, n);
+}
+
+// column tiles of the coupling step: the transformed columns' tiles, at
+// least one per 64 conditioner columns
+__host__ __device__ __forceinline__ int ff_couple_tiles(int Da, int Db) {
+  const int tb = (Db + 63) / 64, ta = (Da + 63) / 64;
+  return tb > ta ? tb : ta;
+}
+
+// forward 3, tile (bx, by) of nt column tiles: [s_raw, t] = h2 W3 + b3 for
+// the transformed columns [j0, j0 + 64) (s at column j, t at Db + j), then
+// the coupling u_b e^s + t with s = 3 tanh(s_raw / 3), S = s, the row sums of
+// s over the tile (ld_part, FP64) and this tile's share of the conditioner
+// columns copied through.  nt may exceed the transformed column tiles (a
+// layer with few transformed columns spreads its conditioner copy wider):
+// the extra tiles only copy.
+__device__ __forceinline__ void ff_couple_tile(int bx, int by, int nt, float* pool, const float* __restrict__ h2,
+                                               int H, const float* __restrict__ W3, const float* __restrict__ b3,
+                                               const int* __restrict__ ia, int Da, const int* __restrict__ ib, int Db,
+                                               int D, const float* __restrict__ U, float* __restrict__ Un,
+                                               float* __restrict__ S, int64_t lds, double* __restrict__ ld_part,
+                                               const int* __restrict__ n_dev, int n) {
+  float* ss = pool + FF_SLABS;  // [64][65]
   const int M = rows_of(n_dev, n);
-  const int m0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  const int m0 = by * 64, j0 = bx * 64;
   if (m0 >= M) return;
-  const int nt = gridDim.x;
   // conditioner columns pass through unchanged
-  const int share = (Da + nt - 1) / nt, ja = blockIdx.x * share, jz = min(Da, ja + share);
+  const int share = (Da + nt - 1) / nt, ja = bx * share, jz = min(Da, ja + share);
+#pragma unroll 4
   for (int i = threadIdx.x; i < 64 * (jz > ja ? jz - ja : 0); i += 256) {
     const int mm = i / (jz - ja), j = ja + i % (jz - ja), m = m0 + mm;
     if (m < M) Un[(int64_t)m * D + ia[j]] = U[(int64_t)m * D + ia[j]];
   }
+  if (j0 >= Db) return;  // a tile past the transformed columns only copies
   const int64_t ld3 = 2 * (int64_t)Db;
   auto af = [&](int mm, int k) { const int m = m0 + mm; return m < M ? h2[(int64_t)m * H + k] : 0.0f; };
   f4 as[2][2], at[2][2];
@@ -317,8 +375,8 @@ __global__ void __launch_bounds__(256) ff_couple(const float* __restrict__ h2, i
   ff_zero(at);
   auto bs = [&](int k, int nn) { const int j = j0 + nn; return j < Db ? W3[k * ld3 + j] : 0.0f; };
   auto bt = [&](int k, int nn) { const int j = j0 + nn; return j < Db ? W3[k * ld3 + Db + j] : 0.0f; };
-  ff_mma<true, true>(af, bs, 0, H, as, As, Bs);
-  ff_mma<true, true>(af, bt, 0, H, at, As, Bs);
+  ff_mma<true, true>(af, bs, 0, H, as, ff_as(pool), ff_bs(pool));
+  ff_mma<true, true>(af, bt, 0, H, at, ff_as(pool), ff_bs(pool));
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
 #pragma unroll
@@ -336,50 +394,76 @@ __global__ void __launch_bounds__(256) ff_couple(const float* __restrict__ h2, i
           Un[(int64_t)m * D + d] = U[(int64_t)m * D + d] * expf(sv) + (at[i][jt][r] + b3[Db + j]);
           S[(int64_t)m * lds + j] = sv;
         }
-        ss[mm][nn] = sv;
+        ss[mm * 65 + nn] = sv;
       }
   __syncthreads();
   if (t < 64 && m0 + t < M) {
     double acc = 0.0;
-    for (int nn = 0; nn < 64; ++nn) acc += (double)ss[t][nn];
-    ld_part[(int64_t)blockIdx.x * n + m0 + t] = acc;
+    for (int nn = 0; nn < 64; ++nn) acc += (double)ss[t * 65 + nn];
+    ld_part[(int64_t)bx * n + m0 + t] = acc;
   }
+  __syncthreads();
 }
 
-// forward end: logdet = sum over layers and tiles of the row sums + sum log
-// scale; X = loc + scale * U_L
+__global__ void __launch_bounds__(256) ff_couple(const float* __restrict__ h2, int H, const float* __restrict__ W3,
+                                                 const float* __restrict__ b3, const int* __restrict__ ia, int Da,
+                                                 const int* __restrict__ ib, int Db, int D,
+                                                 const float* __restrict__ U, float* __restrict__ Un,
+                                                 float* __restrict__ S, int64_t lds, double* __restrict__ ld_part,
+                                                 const int* __restrict__ n_dev, int n) {
+  __shared__ float pool[FF_SLABS + 64 * 65];
+  ff_couple_tile(blockIdx.x, blockIdx.y, gridDim.x, pool, h2, H, W3, b3, ia, Da, ib, Db, D, U, Un, S, lds, ld_part,
+                 n_dev, n);
+}
+
+// forward end for row r: logdet = sum over layers and tiles of the row sums +
+// sum log scale (thread 0); X = loc + scale * U_L over the row
+__device__ __forceinline__ void ff_finish_row(int r, const double* __restrict__ ld_part, const int* __restrict__ Db,
+                                              int L, int tmax, const float* __restrict__ UL,
+                                              const double* __restrict__ loc, const double* __restrict__ scale, int D,
+                                              int n, double log_scale_sum, double* __restrict__ logdet,
+                                              double* __restrict__ X) {
+  // thread t sums the tiles t, t + 256, ... of the layers' tile lists in
+  // order; the four waves' sums are added in a fixed order
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int l = 0; l < L; ++l) {
+    const int nt = (Db[l] + 63) / 64;
+    for (int tt = threadIdx.x; tt < nt; tt += 256) s += ld_part[((int64_t)l * tmax + tt) * n + r];
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+#pragma unroll 4
+  for (int j = threadIdx.x; j < D; j += blockDim.x)
+    X[(int64_t)r * D + j] = loc[j] + scale[j] * (double)UL[(int64_t)r * D + j];
+  __syncthreads();
+  if (threadIdx.x == 0) logdet[r] = ((red[0] + red[1]) + (red[2] + red[3])) + log_scale_sum;
+  __syncthreads();
+}
+
 __global__ void ff_finish(const double* __restrict__ ld_part, const int* __restrict__ Db, int L, int tmax,
                           const float* __restrict__ UL, const double* __restrict__ loc,
                           const double* __restrict__ scale, int D, const int* __restrict__ n_dev, int n,
                           double log_scale_sum, double* __restrict__ logdet, double* __restrict__ X) {
   const int r = blockIdx.x;
   if (r >= rows_of(n_dev, n)) return;
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int l = 0; l < L; ++l) {
-      const int nt = (Db[l] + 63) / 64;
-      for (int tt = 0; tt < nt; ++tt) s += ld_part[((int64_t)l * tmax + tt) * n + r];
-    }
-    logdet[r] = s + log_scale_sum;
-  }
-  for (int j = threadIdx.x; j < D; j += blockDim.x)
-    X[(int64_t)r * D + j] = loc[j] + scale[j] * (double)UL[(int64_t)r * D + j];
+  ff_finish_row(r, ld_part, Db, L, tmax, UL, loc, scale, D, n, log_scale_sum, logdet, X);
 }
 
-// pullback 1 (one layer, transformed columns [j0, j0 + 64)): the coupling
-// cotangents GO = [dU/ds_raw, dU/dt] (dU/ds = gv u e^s - 1, the -1 from the
-// log-det), GC_b <- gv e^s in place, and the partial GP2 = GO W3^T over these
-// columns for every MLP unit
-__global__ void __launch_bounds__(256) fb_couple(float* __restrict__ GC, const float* __restrict__ U,
-                                                 const float* __restrict__ S, int64_t lds,
-                                                 const int* __restrict__ ib, int Db, int D,
-                                                 const float* __restrict__ W3, int H, float* __restrict__ part,
-                                                 const int* __restrict__ n_dev, int n) {
-  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
-  __shared__ float go[64][129];
+// pullback 1, tile (bx, by) (one layer, transformed columns [j0, j0 + 64)):
+// the coupling cotangents GO = [dU/ds_raw, dU/dt] (dU/ds = gv u e^s - 1, the
+// -1 from the log-det), GC_b <- gv e^s in place, and the partial GP2 = GO W3^T
+// over these columns for every MLP unit
+__device__ __forceinline__ void fb_couple_tile(int bx, int by, float* pool, float* __restrict__ GC,
+                                               const float* __restrict__ U, const float* __restrict__ S, int64_t lds,
+                                               const int* __restrict__ ib, int Db, int D,
+                                               const float* __restrict__ W3, int H, float* __restrict__ part,
+                                               const int* __restrict__ n_dev, int n) {
+  float* go = pool + FF_SLABS;  // [64][129]
   const int M = rows_of(n_dev, n);
-  const int m0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  const int m0 = by * 64, j0 = bx * 64;
   if (m0 >= M) return;
+#pragma unroll 8
   for (int i = threadIdx.x; i < 64 * 64; i += 256) {
     const int mm = i >> 6, jj = i & 63, m = m0 + mm, j = j0 + jj;
     float gs = 0.0f, gt = 0.0f;
@@ -393,12 +477,12 @@ __global__ void __launch_bounds__(256) fb_couple(float* __restrict__ GC, const f
       gt = gv;
       GC[e] = gv * es;
     }
-    go[mm][jj] = gs;
-    go[mm][64 + jj] = gt;
+    go[mm * 129 + jj] = gs;
+    go[mm * 129 + 64 + jj] = gt;
   }
   __syncthreads();
   const int64_t ld3 = 2 * (int64_t)Db;
-  auto af = [&](int mm, int k) { return go[mm][k]; };
+  auto af = [&](int mm, int k) { return go[mm * 129 + k]; };
   for (int c0 = 0; c0 < H; c0 += 64) {
     f4 acc[2][2];
     ff_zero(acc);
@@ -409,25 +493,36 @@ __global__ void __launch_bounds__(256) fb_couple(float* __restrict__ GC, const f
       if (c >= H || j >= Db) return 0.0f;
       return W3[(int64_t)c * ld3 + (k < 64 ? j : Db + j)];
     };
-    ff_mma<true, false>(af, bf, 0, 128, acc, As, Bs);
+    ff_mma<true, false>(af, bf, 0, 128, acc, ff_as(pool), ff_bs(pool));
     ff_store(acc, [&](int mm, int nn, float v) {
       const int m = m0 + mm, c = c0 + nn;
-      if (m < M && c < H) part[((int64_t)blockIdx.x * n + m) * H + c] = v;
+      if (m < M && c < H) part[((int64_t)bx * n + m) * H + c] = v;
     });
   }
+  __syncthreads();
 }
 
-// pullback 2: GP2 = (sum of the partials)(1 - h2^2) for the block's rows (all
-// H in LDS), GP1 = (GP2 W2^T)(1 - h1^2) for the block's unit columns
-__global__ void __launch_bounds__(256) fb_mlp(const float* __restrict__ part, int splits,
-                                              const float* __restrict__ h1, const float* __restrict__ h2,
-                                              const float* __restrict__ W2, int H, float* __restrict__ GP1,
-                                              const int* __restrict__ n_dev, int n) {
-  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
-  __shared__ float sg[64][FF_HMAX + 1];
+__global__ void __launch_bounds__(256) fb_couple(float* __restrict__ GC, const float* __restrict__ U,
+                                                 const float* __restrict__ S, int64_t lds,
+                                                 const int* __restrict__ ib, int Db, int D,
+                                                 const float* __restrict__ W3, int H, float* __restrict__ part,
+                                                 const int* __restrict__ n_dev, int n) {
+  __shared__ float pool[FF_SLABS + 64 * 129];
+  fb_couple_tile(blockIdx.x, blockIdx.y, pool, GC, U, S, lds, ib, Db, D, W3, H, part, n_dev, n);
+}
+
+// pullback 2, tile (bx, by): GP2 = (sum of the partials)(1 - h2^2) for the
+// tile's rows (all H in LDS), GP1 = (GP2 W2^T)(1 - h1^2) for the tile's unit
+// columns
+__device__ __forceinline__ void fb_mlp_tile(int bx, int by, float* pool, const float* __restrict__ part, int splits,
+                                            const float* __restrict__ h1, const float* __restrict__ h2,
+                                            const float* __restrict__ W2, int H, float* __restrict__ GP1,
+                                            const int* __restrict__ n_dev, int n) {
+  float* sg = pool + FF_SLABS;  // [64][FF_HMAX + 1]
   const int M = rows_of(n_dev, n);
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int m0 = by * 64, n0 = bx * 64;
   if (m0 >= M) return;
+#pragma unroll 8
   for (int i = threadIdx.x; i < 64 * H; i += 256) {
     const int mm = i / H, c = i % H, m = m0 + mm;
     float v = 0.0f;
@@ -437,15 +532,15 @@ __global__ void __launch_bounds__(256) fb_mlp(const float* __restrict__ part, in
       const float hv = h2[(int64_t)m * H + c];
       v = s * (1.0f - hv * hv);
     }
-    sg[mm][c] = v;
+    sg[mm * (FF_HMAX + 1) + c] = v;
   }
   __syncthreads();
   f4 acc[2][2];
   ff_zero(acc);
-  auto af = [&](int mm, int k) { return sg[mm][k]; };
+  auto af = [&](int mm, int k) { return sg[mm * (FF_HMAX + 1) + k]; };
   // B(k, c) = W2[c][k] (h2 = h1 W2: W2[k_in * H + k_out])
   auto bf = [&](int k, int nn) { const int c = n0 + nn; return c < H ? W2[(int64_t)c * H + k] : 0.0f; };
-  ff_mma<true, false>(af, bf, 0, H, acc, As, Bs);
+  ff_mma<true, false>(af, bf, 0, H, acc, ff_as(pool), ff_bs(pool));
   ff_store(acc, [&](int mm, int nn, float v) {
     const int m = m0 + mm, c = n0 + nn;
     if (m < M && c < H) {
@@ -453,27 +548,42 @@ __global__ void __launch_bounds__(256) fb_mlp(const float* __restrict__ part, in
       GP1[(int64_t)m * H + c] = v * (1.0f - hv * hv);
     }
   });
+  __syncthreads();
 }
 
-// pullback 3: GC[:, ia[j]] += GP1 W1^T for the conditioner columns
-// [j0, j0 + 64) (each element owned by one block)
-__global__ void __launch_bounds__(256) fb_in(const float* __restrict__ GP1, int H, const float* __restrict__ W1,
-                                             const int* __restrict__ ia, int Da, int D, float* __restrict__ GC,
-                                             const int* __restrict__ n_dev, int n) {
-  __shared__ float As[16][FF_LD], Bs[16][FF_LD];
+__global__ void __launch_bounds__(256) fb_mlp(const float* __restrict__ part, int splits,
+                                              const float* __restrict__ h1, const float* __restrict__ h2,
+                                              const float* __restrict__ W2, int H, float* __restrict__ GP1,
+                                              const int* __restrict__ n_dev, int n) {
+  __shared__ float pool[FF_POOL];
+  fb_mlp_tile(blockIdx.x, blockIdx.y, pool, part, splits, h1, h2, W2, H, GP1, n_dev, n);
+}
+
+// pullback 3, tile (bx, by): GC[:, ia[j]] += GP1 W1^T for the conditioner
+// columns [j0, j0 + 64) (each element owned by one tile)
+__device__ __forceinline__ void fb_in_tile(int bx, int by, float* pool, const float* __restrict__ GP1, int H,
+                                           const float* __restrict__ W1, const int* __restrict__ ia, int Da, int D,
+                                           float* __restrict__ GC, const int* __restrict__ n_dev, int n) {
   const int M = rows_of(n_dev, n);
-  const int m0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  const int m0 = by * 64, j0 = bx * 64;
   if (m0 >= M) return;
   f4 acc[2][2];
   ff_zero(acc);
   auto af = [&](int mm, int k) { const int m = m0 + mm; return m < M ? GP1[(int64_t)m * H + k] : 0.0f; };
   // B(k, j) = W1[j][k] (W1 [Da][H])
   auto bf = [&](int k, int nn) { const int j = j0 + nn; return j < Da ? W1[(int64_t)j * H + k] : 0.0f; };
-  ff_mma<true, false>(af, bf, 0, H, acc, As, Bs);
+  ff_mma<true, false>(af, bf, 0, H, acc, ff_as(pool), ff_bs(pool));
   ff_store(acc, [&](int mm, int nn, float v) {
     const int m = m0 + mm, j = j0 + nn;
     if (m < M && j < Da) GC[(int64_t)m * D + ia[j]] += v;
   });
+}
+
+__global__ void __launch_bounds__(256) fb_in(const float* __restrict__ GP1, int H, const float* __restrict__ W1,
+                                             const int* __restrict__ ia, int Da, int D, float* __restrict__ GC,
+                                             const int* __restrict__ n_dev, int n) {
+  __shared__ float pool[FF_SLABS];
+  fb_in_tile(blockIdx.x, blockIdx.y, pool, GP1, H, W1, ia, Da, D, GC, n_dev, n);
 }
 
 // U0[r][j] = Q[idx[r]][j]
@@ -694,6 +804,188 @@ __global__ void flow_adam(float* __restrict__ p, const float* __restrict__ g, fl
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// ------------------------------------------- persistent sampling path
+// The sampler's flow gradient as two launches, forward T(z) and the pullback,
+// each one block of 256 threads per CU running every phase of every layer:
+//   forward   z rows; per layer h1 partials over the conditioner columns,
+//             h1 = tanh(sum + b1), h2 = tanh(h1 W2 + b2), [s, t] = h2 W3 + b3
+//             with the coupling and log-det row sums; X and log|det|
+//   pullback  GC = GX scale; per layer (last first) coupling cotangents and
+//             GP2 partials over the transformed columns, GP2 = their sum,
+//             GP1 = (GP2 (1 - h2^2) W2^T)(1 - h1^2), GC[:, ia] += GP1 W1^T;
+//             G and V of the chains
+// Phases are separated by a grid barrier (fp_sync).  The tiles are the fused
+// per-layer kernels' (ff_*_tile, fb_*_tile) and every sum keeps their fixed
+// order, so the results equal the per-layer kernels' bit for bit.
+#define FP_LMAX 16
+
+struct FpLayer {
+  const int* ia;
+  const int* ib;
+  int Da, Db, kper, splits;  // first product split over the conditioner columns
+  const float *W1, *b1, *W2, *b2, *W3, *b3;
+};
+
+struct FpArgs {
+  FpLayer ly[FP_LMAX];
+  int L, D, H, n, tmax;
+  const int* n_dev;
+  const int* idx;
+  int64_t RD, RH, RS;  // per-layer strides of U, H1/H2, S
+  float *U, *H1, *H2, *S, *part, *GC, *GP1, *GP2;
+  double* ld_part;
+  const int* db_dev;
+  const double *loc, *scale;
+  double log_scale_sum;
+  // forward
+  const double* Q;
+  int64_t ldq;
+  double *X, *logdet;
+  // pullback
+  const double *GX, *VX, *logdet_in;
+  double* G;
+  int64_t ldg;
+  double* V;
+  // barrier: bar[0] arrivals, bar[1] departures; err set when a wait times out
+  unsigned* bar;
+  int* err;
+  uint64_t spin_ticks;  // wall-clock ticks a barrier may wait
+};
+
+// Grid barrier: every wave waits for its stores to be acknowledged, then one
+// thread per block releases at agent scope (an L2 write-back), counts the
+// arrival and polls the count with relaxed loads (an acquiring poll would
+// invalidate the L2 on every read, under every other block's loads) until all
+// blocks of this round have arrived, the error flag is set, or the time limit
+// passes (which sets it); then it acquires once (L1 / L2 invalidation).  The
+// agent-scope write-back and invalidation make the data written before the
+// barrier in one XCD visible after it in another.
+__device__ __forceinline__ void fp_sync(const FpArgs& a, unsigned& target) {
+  target += gridDim.x;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (wall_clock64() - t0 > a.spin_ticks) {
+        __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Kernel end: the last block to leave (every block has finished waiting on
+// bar[0]) resets both counters for the next launch.
+__device__ __forceinline__ void fp_depart(const FpArgs& a) {
+  if (threadIdx.x == 0) {
+    const unsigned d = __hip_atomic_fetch_add(a.bar + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (d == gridDim.x - 1) {
+      __hip_atomic_store(a.bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.bar + 1, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) flow_fwd_persistent(FpArgs a) {
+  __shared__ float pool[FF_POOL];
+  unsigned target = 0;
+  const int M = rows_of(a.n_dev, a.n);
+  const int G = gridDim.x, D = a.D, H = a.H, n = a.n;
+  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x, gs = (int64_t)G * 256;
+#pragma unroll 4
+  for (int64_t i = gt; i < (int64_t)M * D; i += gs) {
+    const int r = (int)(i / D), j = (int)(i % D);
+    const int64_t c = a.idx ? a.idx[r] : r;
+    a.U[i] = (float)a.Q[c * a.ldq + j];
+  }
+  fp_sync(a, target);
+  const int tm = (n + 63) / 64, th = (H + 63) / 64;
+  for (int l = 0; l < a.L; ++l) {
+    const FpLayer& ly = a.ly[l];
+    float* U = a.U + l * a.RD;
+    float* h1 = a.H1 + l * a.RH;
+    float* h2 = a.H2 + l * a.RH;
+    const int S = ly.splits;
+    for (int it = blockIdx.x; it < th * tm * S; it += G)
+      ff_h1_tile(it % th, (it / th) % tm, it / (th * tm), pool, U, D, ly.ia, ly.Da, ly.W1, ly.b1, H, a.n_dev, n,
+                 ly.kper, S, h1, a.part);
+    fp_sync(a, target);
+    if (S > 1) {
+      for (int64_t i = gt; i < (int64_t)M * H; i += gs) {
+        const int m = (int)(i / H), c = (int)(i % H);
+        float s = 0.0f;
+#pragma unroll 8
+        for (int z = 0; z < S; ++z) s += a.part[((int64_t)z * n + m) * H + c];
+        h1[i] = tanhf(s + ly.b1[c]);
+      }
+      fp_sync(a, target);
+    }
+    for (int it = blockIdx.x; it < th * tm; it += G)
+      ff_h2_tile(it % th, it / th, pool, nullptr, 1, ly.b1, h1, ly.W2, ly.b2, h2, H, a.n_dev, n);
+    fp_sync(a, target);
+    const int tc = ff_couple_tiles(ly.Da, ly.Db);
+    for (int it = blockIdx.x; it < tc * tm; it += G)
+      ff_couple_tile(it % tc, it / tc, tc, pool, h2, H, ly.W3, ly.b3, ly.ia, ly.Da, ly.ib, ly.Db, D, U, U + a.RD,
+                     a.S + l * a.RS, (int64_t)ly.Db, a.ld_part + (int64_t)l * a.tmax * n, a.n_dev, n);
+    fp_sync(a, target);
+  }
+  for (int r = blockIdx.x; r < M; r += G)
+    ff_finish_row(r, a.ld_part, a.db_dev, a.L, a.tmax, a.U + a.L * a.RD, a.loc, a.scale, D, n, a.log_scale_sum,
+                  a.logdet, a.X);
+  fp_depart(a);
+}
+
+__global__ void __launch_bounds__(256) flow_bwd_persistent(FpArgs a) {
+  __shared__ float pool[FF_POOL];
+  unsigned target = 0;
+  const int M = rows_of(a.n_dev, a.n);
+  const int G = gridDim.x, D = a.D, H = a.H, n = a.n;
+  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x, gs = (int64_t)G * 256;
+#pragma unroll 4
+  for (int64_t i = gt; i < (int64_t)M * D; i += gs) a.GC[i] = (float)(a.GX[i] * a.scale[i % D]);
+  fp_sync(a, target);
+  const int tm = (n + 63) / 64, th = (H + 63) / 64;
+  for (int l = a.L - 1; l >= 0; --l) {
+    const FpLayer& ly = a.ly[l];
+    const float* h1 = a.H1 + l * a.RH;
+    const float* h2 = a.H2 + l * a.RH;
+    const int tb = (ly.Db + 63) / 64, ta = (ly.Da + 63) / 64;
+    for (int it = blockIdx.x; it < tb * tm; it += G)
+      fb_couple_tile(it % tb, it / tb, pool, a.GC, a.U + l * a.RD, a.S + l * a.RS, (int64_t)ly.Db, ly.ib, ly.Db, D,
+                     ly.W3, H, a.part, a.n_dev, n);
+    fp_sync(a, target);
+    for (int64_t i = gt; i < (int64_t)M * H; i += gs) {
+      const int m = (int)(i / H), c = (int)(i % H);
+      float s = 0.0f;
+#pragma unroll 8
+      for (int z = 0; z < tb; ++z) s += a.part[((int64_t)z * n + m) * H + c];
+      a.GP2[i] = s;
+    }
+    fp_sync(a, target);
+    for (int it = blockIdx.x; it < th * tm; it += G)
+      fb_mlp_tile(it % th, it / th, pool, a.GP2, 1, h1, h2, ly.W2, H, a.GP1, a.n_dev, n);
+    fp_sync(a, target);
+    for (int it = blockIdx.x; it < ta * tm; it += G)
+      fb_in_tile(it % ta, it / ta, pool, a.GP1, H, ly.W1, ly.ia, ly.Da, D, a.GC, a.n_dev, n);
+    fp_sync(a, target);
+  }
+#pragma unroll 4
+  for (int64_t i = gt; i < (int64_t)M * D; i += gs) {
+    const int r = (int)(i / D), j = (int)(i % D);
+    const int64_t c = a.idx ? a.idx[r] : r;
+    a.G[c * a.ldg + j] = (double)a.GC[i];
+    if (j == 0) a.V[c] = a.VX[r] - a.logdet_in[r];
+  }
+  fp_depart(a);
+}
+
 static dim3 grid1(int64_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
 
 #define HIPF(x)                                                                         \
@@ -795,6 +1087,18 @@ FlowNet::FlowNet(const FlowSpec& spec, int max_rows) : spec_(spec), R_(max_rows)
   for (int l = 0; l < L; ++l) dbs[l] = layers_[l].Db;
   ld_part_ = dalloc<double>((int64_t)L * ((Dmax_b_ + 63) / 64) * R);
   db_dev_ = dalloc<int>(L);
+  bar_ = dalloc<unsigned>(2);
+  bar_err_ = dalloc<int>(1);
+  HIPF(hipMemset(bar_, 0, 2 * sizeof(unsigned)));
+  HIPF(hipMemset(bar_err_, 0, sizeof(int)));
+  {
+    int dev = 0, cus = 0, khz = 0;
+    HIPF(hipGetDevice(&dev));
+    HIPF(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPF(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    n_cu_ = cus;
+    spin_ticks_ = (uint64_t)std::max(khz, 1000) * 2000;  // 2 s
+  }
   HIPF(hipMemcpy(db_dev_, dbs.data(), L * sizeof(int), hipMemcpyHostToDevice));
   std::vector<double> z(D, 0.0), o(D, 1.0);
   set_standardization(z.data(), o.data());
@@ -810,7 +1114,7 @@ FlowNet::~FlowNet() {
   for (void* p : {(void*)params_, (void*)grad_, (void*)adam_m_, (void*)adam_v_, (void*)loc_, (void*)scale_, (void*)U_,
                   (void*)H1_, (void*)H2_, (void*)O_, (void*)S_, (void*)GO_, (void*)GP2_, (void*)GP1_,
                   (void*)GC_, (void*)part_, (void*)rowbuf_, (void*)rowbuf2_, (void*)pick_, (void*)xrows_,
-                  (void*)db_dev_, (void*)ld_part_})
+                  (void*)db_dev_, (void*)ld_part_, (void*)bar_, (void*)bar_err_})
     if (p) (void)hipFree(p);
 }
 
@@ -915,11 +1219,70 @@ bool FlowNet::fused_ok(int n_rows) const {
   return spec_.H <= FF_HMAX && (int64_t)((Dmax_b_ + 63) / 64) * rh <= part_n_;
 }
 
+// The persistent kernels run the sampling path with CMDSTAN_AMD_FLOW_PERSISTENT=1
+// (CMDSTAN_AMD_FLOW_FUSED=1, the per-layer kernels, takes precedence) when
+// they apply: MLP width within the LDS row buffers, at most FP_LMAX layers,
+// the first product's and the pullback's partial sums within the workspace.
+// A launch is one block per CU (about 100 KB of LDS each), all resident at
+// once, which the grid barriers need.
+bool FlowNet::persistent_ok(int n_rows) const {
+  const char* e = std::getenv("CMDSTAN_AMD_FLOW_PERSISTENT");
+  if (!e || std::atoi(e) == 0) return false;
+  if (spec_.H > FF_HMAX || spec_.L > FP_LMAX || n_cu_ < 1) return false;
+  const int64_t rh = (int64_t)n_rows * spec_.H;
+  for (const Layer& ly : layers_) {
+    int kper = 0;
+    const int splits = ff_h1_splits(ly.Da, spec_.H, n_rows, part_n_, &kper);
+    if ((int64_t)splits * rh > part_n_ || (int64_t)((ly.Db + 63) / 64) * rh > part_n_) return false;
+  }
+  return true;
+}
+
+void FlowNet::persistent_args(FpArgs& a, const int* idx, const int* n_dev, int n_rows) const {
+  a = FpArgs{};
+  const int D = spec_.D, L = spec_.L, H = spec_.H;
+  for (int l = 0; l < L; ++l) {
+    const Layer& ly = layers_[l];
+    FpLayer& f = a.ly[l];
+    f.ia = ly.ia; f.ib = ly.ib; f.Da = ly.Da; f.Db = ly.Db;
+    f.splits = ff_h1_splits(ly.Da, H, n_rows, part_n_, &f.kper);
+    f.W1 = params_ + ly.off[0]; f.b1 = params_ + ly.off[1]; f.W2 = params_ + ly.off[2];
+    f.b2 = params_ + ly.off[3]; f.W3 = params_ + ly.off[4]; f.b3 = params_ + ly.off[5];
+  }
+  a.L = L; a.D = D; a.H = H; a.n = n_rows; a.tmax = (Dmax_b_ + 63) / 64;
+  a.n_dev = n_dev; a.idx = idx;
+  a.RD = (int64_t)R_ * D; a.RH = (int64_t)R_ * H; a.RS = (int64_t)R_ * Dmax_b_;
+  a.U = U_; a.H1 = H1_; a.H2 = H2_; a.S = S_; a.part = part_; a.GC = GC_; a.GP1 = GP1_; a.GP2 = GP2_;
+  a.ld_part = ld_part_; a.db_dev = db_dev_; a.loc = loc_; a.scale = scale_; a.log_scale_sum = log_scale_sum_;
+  a.bar = bar_; a.err = bar_err_; a.spin_ticks = spin_ticks_;
+}
+
+void FlowNet::check_barriers() {
+  if (!ran_persistent_) return;
+  int e = 0;
+  HIPF(hipMemcpyAsync(&e, bar_err_, sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPF(hipStreamSynchronize(st_));
+  if (e) {
+    HIPF(hipMemsetAsync(bar_, 0, 2 * sizeof(unsigned), st_));
+    HIPF(hipMemsetAsync(bar_err_, 0, sizeof(int), st_));
+    HIPF(hipStreamSynchronize(st_));
+    throw std::runtime_error("flow: a grid barrier of the persistent sampling kernels timed out");
+  }
+}
+
 void FlowNet::forward(const double* Q, int64_t ldq, const int* idx, const int* n_dev, int n_rows, double* X,
                       double* logdet) {
   if (n_rows > R_) throw std::invalid_argument("flow: too many rows");
   const int D = spec_.D, L = spec_.L, H = spec_.H;
   const int64_t RD = (int64_t)R_ * D;
+  if (!fused_ok(n_rows) && persistent_ok(n_rows)) {
+    FpArgs a;
+    persistent_args(a, idx, n_dev, n_rows);
+    a.Q = Q; a.ldq = ldq; a.X = X; a.logdet = logdet;
+    hipLaunchKernelGGL(flow_fwd_persistent, dim3(n_cu_), dim3(256), 0, st_, a);
+    ran_persistent_ = true;
+    return;
+  }
   hipLaunchKernelGGL(flow_gather_z, grid1((int64_t)n_rows * D), dim3(256), 0, st_, Q, ldq, idx, n_dev, n_rows, D, U_);
   if (fused_ok(n_rows)) {
     const float* P = params_;
@@ -935,7 +1298,7 @@ void FlowNet::forward(const double* Q, int64_t ldq, const int* idx, const int* n
                          P + ly.off[1], H, n_dev, n_rows, kper, splits, h1, part_);
       hipLaunchKernelGGL(ff_h2, dim3(th, tm), dim3(256), 0, st_, part_, splits, P + ly.off[1], h1, P + ly.off[2],
                          P + ly.off[3], h2, H, n_dev, n_rows);
-      hipLaunchKernelGGL(ff_couple, dim3((ly.Db + 63) / 64, tm), dim3(256), 0, st_, h2, H, P + ly.off[4],
+      hipLaunchKernelGGL(ff_couple, dim3(ff_couple_tiles(ly.Da, ly.Db), tm), dim3(256), 0, st_, h2, H, P + ly.off[4],
                          P + ly.off[5], ly.ia, ly.Da, ly.ib, ly.Db, D, U, U + RD, S_ + (int64_t)l * R_ * Dmax_b_,
                          (int64_t)ly.Db, ld_part_ + (int64_t)l * tmax * n_rows, n_dev, n_rows);
     }
@@ -960,6 +1323,14 @@ void FlowNet::pullback(const double* GX, const double* VX, const double* logdet,
                        int n_rows, double* G, int64_t ldg, double* V) {
   const int D = spec_.D, L = spec_.L, H = spec_.H;
   const int64_t RD = (int64_t)R_ * D;
+  if (!fused_ok(n_rows) && persistent_ok(n_rows)) {
+    FpArgs a;
+    persistent_args(a, idx, n_dev, n_rows);
+    a.GX = GX; a.VX = VX; a.logdet_in = logdet; a.G = G; a.ldg = ldg; a.V = V;
+    hipLaunchKernelGGL(flow_bwd_persistent, dim3(n_cu_), dim3(256), 0, st_, a);
+    ran_persistent_ = true;
+    return;
+  }
   hipLaunchKernelGGL(flow_bwd_init, grid1((int64_t)n_rows * D), dim3(256), 0, st_, GX, scale_, D, n_dev, n_rows, GC_);
   if (fused_ok(n_rows)) {
     const float* P = params_;

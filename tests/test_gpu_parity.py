@@ -441,7 +441,11 @@ def test_interrupt_callback_stops_the_run():
     assert np.all(fit.status == 0)
 
 
-def test_graph_replay_matches_eager_with_flow(monkeypatch):
+@pytest.mark.parametrize("persistent", ["0", "1"])
+def test_graph_replay_matches_eager_with_flow(monkeypatch, persistent):
+    """Graph replay of the lockstep bundles with a flow, on the GEMM chain and
+    on the persistent flow kernels: draws identical to eager launches."""
+    monkeypatch.setenv("CMDSTAN_AMD_FLOW_PERSISTENT", persistent)
     from tests.model_cases import gauss
     m = ca.Model("gauss", gauss(seed=6, D=6))
     kw = dict(num_chains=32, num_warmup=150, num_samples=60, seed=2, flow="force", flow_layers=2, flow_hidden=16,

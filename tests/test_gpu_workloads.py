@@ -86,13 +86,23 @@ def test_c4_batched_gradient_256_points():
     assert worst <= 1e-12
 
 
-@pytest.mark.parametrize("fused", ["0", "1"])
-def test_c4_flow_forward_and_pullback_d4020(monkeypatch, fused):
+_FLOW_KERNELS = {"gemm": ("0", "0"), "layers": ("1", "0"), "persistent": ("0", "1")}
+
+
+def _flow_kernels(monkeypatch, kernels):
+    fused, persistent = _FLOW_KERNELS[kernels]
+    monkeypatch.setenv("CMDSTAN_AMD_FLOW_FUSED", fused)
+    monkeypatch.setenv("CMDSTAN_AMD_FLOW_PERSISTENT", persistent)
+
+
+@pytest.mark.parametrize("kernels", ["gemm", "layers", "persistent"])
+def test_c4_flow_forward_and_pullback_d4020(monkeypatch, kernels):
     """The C4 flow (8 affine-coupling layers, MLP 2 x 128, masks with the 20
     hyperparameters as globals and groups of 10) at D = 4020, 256 rows, against
-    the FP64 oracle at the FP32 tolerance 2e-4, on the split-K GEMM chain and
-    on the fused per-layer kernels (CMDSTAN_AMD_FLOW_FUSED=1)."""
-    monkeypatch.setenv("CMDSTAN_AMD_FLOW_FUSED", fused)
+    the FP64 oracle at the FP32 tolerance 2e-4, on the split-K GEMM chain, on
+    the fused per-layer kernels (CMDSTAN_AMD_FLOW_FUSED=1) and on the two
+    persistent kernels (CMDSTAN_AMD_FLOW_PERSISTENT=1)."""
+    _flow_kernels(monkeypatch, kernels)
     D, L, H, ng, gs = 4020, 8, 128, 20, 10
     g = ca.Flow(D, n_layers=L, hidden=H, n_glob=ng, group_size=gs, max_rows=256, seed=3)
     rng = np.random.default_rng(3)
@@ -110,6 +120,28 @@ def test_c4_flow_forward_and_pullback_d4020(monkeypatch, fused):
     gx = rng.standard_normal((256, D))
     gz, gzo = g.pullback_grad(z, gx), o.pullback_grad(z, gx)
     assert np.max(np.abs(gz - gzo)) <= 2e-4 * max(1.0, np.max(np.abs(gzo)))
+
+
+@pytest.mark.parametrize("rows", [256, 100, 1])
+def test_c4_flow_persistent_equals_layer_kernels(monkeypatch, rows):
+    """The persistent forward / pullback kernels run the per-layer kernels'
+    tiles with the same summation order: bit-identical results at the C4
+    shape, for full, ragged and single-row batches."""
+    D, L, H, ng, gs = 4020, 8, 128, 20, 10
+    rng = np.random.default_rng(5)
+    z = rng.standard_normal((rows, D))
+    gx = rng.standard_normal((rows, D))
+    out = {}
+    for kernels in ("layers", "persistent"):
+        _flow_kernels(monkeypatch, kernels)
+        g = ca.Flow(D, n_layers=L, hidden=H, n_glob=ng, group_size=gs, max_rows=256, seed=3)
+        p = g.get_params().astype(np.float64)
+        g.set_params((p + 0.05 * np.random.default_rng(6).standard_normal(p.size)).astype(np.float32))
+        x, ld = g.forward(z)
+        gz = g.pullback_grad(z, gx)
+        out[kernels] = (x, ld, gz)
+    for a, b in zip(out["layers"], out["persistent"]):
+        assert np.array_equal(a, b)
 
 
 def _bf16_round(x):

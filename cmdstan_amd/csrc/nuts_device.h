@@ -23,8 +23,8 @@
 // Minv·p enters the leapfrog and the U-turn checks.  For unit/diag it is formed
 // on the fly; for dense_e every momentum kept for a U-turn check has a stored
 // sharp companion, computed once per leaf by a matvec with the per-chain D×D
-// inverse metric, and momenta are drawn by the triangular solve Lᵀ p = z with
-// the chain's Cholesky factor Minv = L Lᵀ.
+// inverse metric, and momenta p = L⁻ᵀ z are drawn with the chain's Cholesky
+// factor Minv = L Lᵀ, whose buffer also holds L⁻¹ (see dense_factor).
 #pragma once
 #include "device_common.h"
 #include "nuts_types.h"
@@ -37,8 +37,8 @@ struct ExternalGrad {
   __device__ double operator()(int, const T*, T*, int, int) const { return 0.0; }
 };
 
-// dense_e keeps one momentum in an LDS strip of the wave during the
-// triangular solve: D <= 64 * DENSE_RMAX
+// dense_e keeps the normal draws of one momentum in an LDS strip of the wave
+// while it is formed: D <= 64 * DENSE_RMAX
 #define DENSE_RMAX 16
 
 // Makes this wave's earlier global writes visible to later reads by other
@@ -362,12 +362,116 @@ __device__ __noinline__ void dense_matvec2(const double* __restrict__ M, const V
   }
 }
 
+// p = L⁻ᵀ z and ps = Minv p = L Lᵀ L⁻ᵀ z = L z in one pass over the packed
+// factor F (column-major D×D: L on and below the diagonal, row r of L⁻¹ in
+// column r above it, so F[j][i] (column j, row i < j) = (L⁻¹)[j][i] =
+// (L⁻ᵀ)[i][j]).  z is in LDS; the lane owning row i sums column j's element
+// into p_i when i < j and into ps_i when i > j; the diagonal terms z_i / L_ii
+// and L_ii z_i come last.  Every element of F is read once, columns
+// coalesced, with no sequential dependency between rows.  Inlined, not an
+// out-of-line call like the matvecs: z lives in the caller's LDS strip, and
+// an out-of-line copy that every caller passes the same strip was built with
+// that strip's address folded in as LDS offset 0, which is not where the
+// calling kernel placed it.
+template <int TPC, class VT>
+__device__ __forceinline__ void dense_tri_sample(const double* __restrict__ F, const double* __restrict__ z,
+                                              VT* __restrict__ p, VT* __restrict__ ps, int D, int tid) {
+  constexpr int RB = 4;  // rows per lane per pass: inlined, so few registers
+  for (int i0 = 0; i0 < D; i0 += TPC * RB) {
+    double a1[RB], a2[RB];
+    int ii[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      a1[r] = 0.0;
+      a2[r] = 0.0;
+      const int i = i0 + tid + TPC * r;
+      ii[r] = i < D ? i : D - 1;
+    }
+    int j = 0;
+    for (; j + 4 <= D; j += 4) {
+      double v[4][RB], zz[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        zz[u] = z[j + u];
+        const double* col = F + (int64_t)(j + u) * D;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) v[u][r] = col[ii[r]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          const double t = v[u][r] * zz[u];
+          a1[r] += ii[r] < j + u ? t : 0.0;
+          a2[r] += ii[r] > j + u ? t : 0.0;
+        }
+    }
+    for (; j < D; ++j) {
+      const double zj = z[j];
+      const double* col = F + (int64_t)j * D;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const double t = col[ii[r]] * zj;
+        a1[r] += ii[r] < j ? t : 0.0;
+        a2[r] += ii[r] > j ? t : 0.0;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int i = i0 + tid + TPC * r;
+      if (i < D) {
+        const double lii = F[(int64_t)i * D + i];
+        p[i] = a1[r] + z[i] / lii;
+        ps[i] = a2[r] + lii * z[i];
+      }
+    }
+  }
+}
+
 // col += w * d (the dense Welford covariance update, one column per call)
 template <int TPC, class VT>
 __device__ __forceinline__ void pass_axpy(double* __restrict__ col, const VT* __restrict__ w, double d, int D,
                                           int tid) {
 #pragma unroll 4
   for (int i = tid; i < D; i += TPC) col[i] = col[i] + (double)w[i] * d;
+}
+
+// X = L⁻¹ (lower triangular) into the strictly upper part of the factor's
+// buffer, row r of X in column r: X[r][c] (c < r) at L[r D + c].  Row r by
+// forward substitution, X[r][c] = -(Σ_{k=c}^{r-1} L[r][k] X[k][c]) / L[r][r]
+// with X[c][c] = 1 / L[c][c]; the lane owning column c reads only its own
+// earlier results and the lower triangle, so rows need no barrier between
+// them.  k runs in the same order on every lane: L[r][k] is one broadcast
+// read and X[k][c] a coalesced one.  Four partial sums per column.  Kept out
+// of line: with this loop inlined into the window-end path, the 256-thread
+// state machine's single-transition replays (which never reach that path) no
+// longer matched the oracle, while this out-of-line form matches; the cause
+// was not found.
+template <int TPC>
+__device__ __noinline__ void dense_inverse(double* L, int D, int tid) {
+  for (int r = 1; r < D; ++r) {
+    const double lrr = L[(int64_t)r * D + r];
+    for (int c0 = 0; c0 < r; c0 += TPC) {
+      const int c = c0 + tid;
+      const bool own = c < r;
+      const int cc = own ? c : 0;
+      // k = c term with X[c][c] = 1 / L[c][c]
+      double a0 = own ? L[(int64_t)cc * D + r] / L[(int64_t)cc * D + cc] : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int k = c0 + 1;
+      for (; k + 4 <= r; k += 4) {
+        const double l0 = L[(int64_t)k * D + r], l1 = L[(int64_t)(k + 1) * D + r];
+        const double l2 = L[(int64_t)(k + 2) * D + r], l3 = L[(int64_t)(k + 3) * D + r];
+        const double x0 = L[(int64_t)k * D + cc], x1 = L[(int64_t)(k + 1) * D + cc];
+        const double x2 = L[(int64_t)(k + 2) * D + cc], x3 = L[(int64_t)(k + 3) * D + cc];
+        a0 += k > cc ? l0 * x0 : 0.0;
+        a1 += k + 1 > cc ? l1 * x1 : 0.0;
+        a2 += k + 2 > cc ? l2 * x2 : 0.0;
+        a3 += k + 3 > cc ? l3 * x3 : 0.0;
+      }
+      for (; k < r; ++k) a0 += k > cc ? L[(int64_t)k * D + r] * L[(int64_t)k * D + cc] : 0.0;
+      if (own) L[(int64_t)r * D + c] = -((a0 + a1) + (a2 + a3)) / lrr;
+    }
+  }
 }
 
 template <int METRIC, class GradFn, int TPC = NUTS_WAVE, class VT = double>
@@ -530,27 +634,17 @@ struct NutsWave {
       advance_normals();
       return;
     }
-    // back substitution Lᵀ p = z (U = Lᵀ upper) in an LDS strip owned by this
-    // wave; column i of L below the diagonal is contiguous in the
-    // column-major factor
+    // the reference solves Lᵀ p = z; p = L⁻ᵀ z with the stored inverse is the
+    // same vector without D dependent steps, and Minv p = L z comes with it
     __shared__ double sp[4][NUTS_WAVE * DENSE_RMAX];
-    double* pw = sp[TPC == NUTS_WAVE ? (threadIdx.x >> 6) : 0];
-    for (int i = tid; i < D; i += TPC) pw[i] = normal_draw(i); converge();
+    double* zw = sp[TPC == NUTS_WAVE ? (threadIdx.x >> 6) : 0];
+    for (int i = tid; i < D; i += TPC) zw[i] = normal_draw(i); converge();
     advance_normals();
     lsync();
-    for (int i = D - 1; i >= 0; --i) {
-      const double* Li = dL + (int64_t)i * D;
-      double a = 0;
-      for (int j = i + 1 + tid; j < D; j += TPC) a += Li[j] * pw[j]; converge();
-      a = gsum(a);
-      const double pi = (pw[i] - a) / Li[i];
-      lsync();  // every lane has read z_i before it is overwritten
-      if (tid == 0) pw[i] = pi; converge();
-      lsync();
-    }
-    for (int i = tid; i < D; i += TPC) p[i] = pw[i]; converge();
+    dense_tri_sample<TPC, VT>(dL, zw, p, ps, D, tid);
+    converge();
+    lsync();  // the strip is free for the next draw
     gsync();
-    matvec(p, ps);
   }
 
   // ------------------------------------------------------------- vectors
@@ -1283,10 +1377,11 @@ struct NutsWave {
         for (int k = 0; k < j; ++k) v -= L[(int64_t)k * D + i] * L[(int64_t)k * D + j];
         Lj[i] = v / d;
       } converge();
-      for (int i = tid; i < j; i += TPC) Lj[i] = 0.0; converge();
       if (tid == 0) Lj[j] = d; converge();
       gsync();
     }
+    dense_inverse<TPC>(L, D, tid);
+    gsync();
     return true;
   }
 };

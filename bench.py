@@ -118,6 +118,15 @@ def cpu_model():
     return "unknown"
 
 
+def default_sizes(config):
+    """(steps, warmup) when the flags are not given: a run without flags
+    finishes in a few minutes and measures the configuration the documented
+    figures are quoted on (C2: 20 steps of 50 iterations after 1000 warmup
+    iterations)."""
+    return {"c1": (1000, 1000), "c2": (20, 5), "c3": (200, 300), "c4": (300, 500),
+            "c4f": (30, 0), "c5": (100, 150)}[config]
+
+
 def step_iterations(steps):
     """Sampling iterations per step: K steps hold at least 1000 draws."""
     return max(50, -(-1000 // max(1, steps)))
@@ -463,14 +472,22 @@ def bench_c5(args):
             "config": {"workload": f"logistic regression N={N} D={D}, {C} chains, dense_e, X stored {prec}"},
             "grad_evals_per_sec": grads / fit.sampling_seconds, "min_bulk_ess": ess,
             "rhat_max": float(np.nanmax(st[:, 9])), "sampling_seconds": fit.sampling_seconds,
-            "warmup_seconds": fit.warmup_seconds, "roofline": roof}
+            "warmup_seconds": fit.warmup_seconds,
+            "autotuned_state_machine": {"threads_per_chain": fit.tuned_threads_per_chain,
+                                        "median_launch_us_wave": fit.tuned_launch_seconds[0] * 1e6,
+                                        "median_launch_us_block": fit.tuned_launch_seconds[1] * 1e6},
+            "roofline": roof}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000, help="timed sampling iterations")
-    ap.add_argument("--warmup", type=int, default=1000, help="untimed warmup (adaptation) iterations")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps: c2 steps of at least 50 iterations holding >= 1000 draws (default 20); "
+                         "other configs: sampling iterations (defaults c1 1000, c3 200, c4 300, c5 100, c4f 30)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed warmup: c2 steps (default 5; at least 1000 iterations); other configs: "
+                         "iterations (defaults c1 1000, c3 300, c4 500, c5 150)")
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--N", type=int, default=100000)
     ap.add_argument("--D", type=int, default=100)
@@ -494,6 +511,11 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run a device copy of BYTES (counter calibration for profiles)")
     args = ap.parse_args()
+    steps0, warm0 = default_sizes(args.config)
+    if args.steps is None:
+        args.steps = steps0
+    if args.warmup is None:
+        args.warmup = warm0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the chain-sharded multi-GPU design is documented "
                                    "but not built", "n_gpus": args.gpus}))

@@ -888,7 +888,11 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
     }
   } ev;
   EvPairs evs;  // around the state-machine launches of the tuning intervals
-  int tune_left = tune_threads ? 2 : 0;
+  // four timed intervals in the order wave, block, block, wave: each mode's
+  // figure is the mean of its two interval medians, so a launch cost that
+  // changes steadily over the first intervals (the step size and the trees
+  // grow through early warmup) weighs on both modes alike
+  int tune_left = tune_threads ? 4 : 0;
   double tune_s[2] = {0, 0};
   DevBuf<long long> tally;
   if (!fused) {
@@ -1000,7 +1004,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         if (!(blk && launch_advance_fused_block(M.kind, cfg.metric, vec32, P, B, md, st)))
           launch_advance_fused(M.kind, cfg.metric, vec32, P, B, md, st);
       } else if (tune_left > 0) {
-        tuning = tune_left == 2 ? 0 : 1;
+        tuning = (tune_left == 4 || tune_left == 1) ? 0 : 1;
         block_per_chain = tuning == 1;
         enqueue_bundle(true, true);
         res.leaf_steps += kCheck;
@@ -1051,7 +1055,7 @@ RunResult run_batch(ModelImpl& M, const RunSpec& R) {
         float t[kCheck - 1];
         for (int k = 1; k < kCheck; ++k) HIPC(hipEventElapsedTime(&t[k - 1], evs.a[k], evs.b[k]));
         std::nth_element(t, t + (kCheck - 1) / 2, t + kCheck - 1);
-        tune_s[tuning] = t[(kCheck - 1) / 2] * 1e-3;
+        tune_s[tuning] += 0.5 * t[(kCheck - 1) / 2] * 1e-3;
         if (--tune_left == 0) {
           block_per_chain = default_block ? !(tune_s[0] < 0.8 * tune_s[1]) : (tune_s[1] < 0.8 * tune_s[0]);
           res.wave_launch_s = tune_s[0];

@@ -259,9 +259,10 @@ typedef struct nuts_output {
   int* init_rejections;
   /* lockstep state machine (no counterpart in CmdStan): threads per chain the
    * run settled on -- 64 (one wave) or 256 (one block), timed on the first
-   * check intervals when neither the caller nor CMDSTAN_AMD_BLOCK_PER_CHAIN
-   * fixes it -- and the median time of one state-machine launch in each
-   * mode over those intervals (0 when not timed; fused mode reports 0 threads) */
+   * four check intervals (wave, block, block, wave) when neither the caller
+   * nor CMDSTAN_AMD_BLOCK_PER_CHAIN fixes it -- and the time of one
+   * state-machine launch in each mode, the mean of its two intervals' medians
+   * (0 when not timed; fused mode reports 0 threads) */
   int tuned_threads_per_chain;
   double tuned_wave_launch_seconds;
   double tuned_block_launch_seconds;

@@ -135,6 +135,23 @@ def test_bench_refuses_more_than_one_gpu():
     assert out["n_gpus"] == 2 and "one GPU" in out["error"]
 
 
+def test_bench_defaults_finish_in_minutes():
+    """Without --steps / --warmup the bench measures the documented sizes: C2
+    1000 warmup + 1000 draws (20 steps of 50 iterations), the other configs
+    their quoted iteration counts."""
+    import importlib.util
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_defaults", os.path.join(root, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    steps, warm = b.default_sizes("c2")
+    S = b.step_iterations(steps)
+    assert steps * S == 1000 and max(1000, warm * S) == 1000
+    assert b.default_sizes("c5") == (100, 150) and b.default_sizes("c3") == (200, 300)
+    assert b.default_sizes("c1") == (1000, 1000)
+
+
 def test_precision_fp32_config_validates():
     """precision 1 is accepted by the config check (FP32 contraction for the
     logistic regression, FP32 leapfrog on the per-chain kernel); 3 is not."""

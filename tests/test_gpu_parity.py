@@ -310,7 +310,8 @@ def test_reduced_precision_posterior_matches_fp64(precision, path, monkeypatch):
 
 def test_state_machine_threads_per_chain_tuned(monkeypatch):
     """Lockstep runs time the wave-per-chain and the block-per-chain state
-    machine on their first two check intervals and report both; the other
+    machine on their first four check intervals (wave, block, block, wave)
+    and report both; the other
     mode replaces the default by dimension (wave below D = 512) only when it is
     more than 20% faster.  CMDSTAN_AMD_BLOCK_PER_CHAIN pins the mode and turns
     the timing off; fused runs report 0 threads."""

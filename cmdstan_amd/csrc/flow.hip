@@ -150,6 +150,8 @@ __global__ void flow_gemm_reduce(GemmDesc g) {
   if (i >= (int64_t)M * g.N) return;
   const int m = (int)(i / g.N), n = (int)(i % g.N);
   float s = 0.0f;
+  // split order kept; eight loads in flight instead of one
+#pragma unroll 8
   for (int sp = 0; sp < g.splits; ++sp) s += g.part[((int64_t)sp * g.M + m) * g.N + n];
   g.C[fg_cidx(g, m, n)] = fg_epi(g, s, m, n);
 }
@@ -308,6 +310,7 @@ __device__ __forceinline__ void ff_h2_tile(int bx, int by, float* pool, const fl
         v = h1[(int64_t)m * H + c];
       } else {
         float s = 0.0f;
+#pragma unroll 8
         for (int z = 0; z < splits; ++z) s += part[((int64_t)z * n + m) * H + c];
         v = tanhf(s + b1[c]);
         if (bx == 0) h1[(int64_t)m * H + c] = v;
@@ -528,6 +531,7 @@ __device__ __forceinline__ void fb_mlp_tile(int bx, int by, float* pool, const f
     float v = 0.0f;
     if (m < M) {
       float s = 0.0f;
+#pragma unroll 8
       for (int z = 0; z < splits; ++z) s += part[((int64_t)z * n + m) * H + c];
       const float hv = h2[(int64_t)m * H + c];
       v = s * (1.0f - hv * hv);
@@ -755,6 +759,8 @@ __global__ void flow_colsum(const float* __restrict__ A, int rows, int cols, flo
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
   float s = 0.0f;
+  // row order kept; eight loads in flight instead of one
+#pragma unroll 8
   for (int r = 0; r < rows; ++r) s += A[(int64_t)r * cols + c];
   out[c] = s;
 }

@@ -862,7 +862,7 @@ struct FpArgs {
 // thread per block releases at agent scope (an L2 write-back), counts the
 // arrival and polls the count with relaxed loads (an acquiring poll would
 // invalidate the L2 on every read, under every other block's loads) until all
-// blocks of this round have arrived, the error flag is set, or the time limit
+// blocks have arrived at this barrier, the error flag is set, or the time limit
 // passes (which sets it); then it acquires once (L1 / L2 invalidation).  The
 // agent-scope write-back and invalidation make the data written before the
 // barrier in one XCD visible after it in another.

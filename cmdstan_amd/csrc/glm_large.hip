@@ -144,6 +144,15 @@ __global__ void __launch_bounds__(256) gl_eta_kernel(GlLargeArgs a, const XT* __
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
   double lpc[4] = {0, 0, 0, 0};  // per n-tile j, this lane's column
+  // this lane's sixteen rows' outcomes, read once (not once per chain tile)
+  float yv[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wr + 16 * i + 4 * (l >> 4) + r;
+      yv[i][r] = m < a.N ? (float)a.y[m] : 0.0f;
+    }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -154,11 +163,12 @@ __global__ void __launch_bounds__(256) gl_eta_kernel(GlLargeArgs a, const XT* __
         const int n = n0 + wc + 16 * j + (l & 15);
         if (m < a.N && n < n_act) {
           const float e = acc[i][j][r];
-          const float yy = (float)a.y[m];
+          const float yy = yv[i][r];
           const float te = __expf(-fabsf(e));
           const float l1p = te < 1e-4f ? te * (1.0f - 0.5f * te) : __logf(1.0f + te);
           lpc[j] += (double)(yy * e - ((e > 0.0f ? e : 0.0f) + l1p));
-          a.R[(int64_t)m * a.ldr + n] = (e >= 0.0f ? 1.0f : te) / (1.0f + te) - yy;
+          // sigmoid(e) - y with the hardware reciprocal of 1 + t in [1, 2]
+          a.R[(int64_t)m * a.ldr + n] = (e >= 0.0f ? 1.0f : te) * __builtin_amdgcn_rcpf(1.0f + te) - yy;
         }
       }
   // lanes l, l^16, l^32, l^48 hold the same column

@@ -162,6 +162,15 @@ __global__ void __launch_bounds__(256, 2) gbl_eta_kernel(GlLargeArgs a, GlLargeP
   const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
   const int64_t plane = (int64_t)p.Cp * p.ldn;
   double lpc[4] = {0, 0, 0, 0};
+  // this lane's sixteen rows' outcomes, read once (not once per chain tile)
+  float yv[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wr + 16 * i + 4 * lg + r;
+      yv[i][r] = m < a.N ? (float)a.y[m] : 0.0f;
+    }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int n = n0 + wc + 16 * j + lr;  // chain position
@@ -175,11 +184,12 @@ __global__ void __launch_bounds__(256, 2) gbl_eta_kernel(GlLargeArgs a, GlLargeP
         float rr = 0.0f;
         if (m < a.N && n < n_act) {
           const float e = acc[i][j][r];
-          const float yy = (float)a.y[m];
+          const float yy = yv[i][r];
           const float te = __expf(-fabsf(e));
           const float l1p = te < 1e-4f ? te * (1.0f - 0.5f * te) : __logf(1.0f + te);
           lpc[j] += (double)(yy * e - ((e > 0.0f ? e : 0.0f) + l1p));
-          rr = (e >= 0.0f ? 1.0f : te) / (1.0f + te) - yy;
+          // sigmoid(e) - y with the hardware reciprocal of 1 + t in [1, 2]
+          rr = (e >= 0.0f ? 1.0f : te) * __builtin_amdgcn_rcpf(1.0f + te) - yy;
         }
         gbl_split3(rr, t0[r], t1[r], t2[r]);
       }

@@ -83,13 +83,34 @@ __device__ __forceinline__ double atanh2_series(double s) {
 }
 __device__ __forceinline__ double log1p_unit(double t) { return atanh2_series(t / (2.0 + t)); }
 
-// 1/d for d in [1, 3]: the hardware reciprocal estimate refined by two Newton
+// 1/d for d in [1, 8]: the hardware reciprocal estimate refined by two Newton
 // steps (no range reduction is needed in this range); within an ulp or two of
 // the correctly rounded quotient, at about half the instructions of a division
-__device__ __forceinline__ double recip_1to3(double d) {
+__device__ __forceinline__ double recip_small(double d) {
   double x = __builtin_amdgcn_rcp(d);
   x = __builtin_fma(x, __builtin_fma(-d, x, 1.0), x);
   return __builtin_fma(x, __builtin_fma(-d, x, 1.0), x);
+}
+
+// log(1 + t) and r1 = 1/(1 + t) for t in [0, 1] from one reciprocal.  Above
+// t = sqrt(2) - 1 the argument is halved, log(1 + t) = log 2 + 2 atanh(s) with
+// s = (t - 1) / (t + 3); below it s = t / (2 + t).  Either way |s| <=
+// 3 - 2 sqrt(2) = 0.1716, and 2 (s + s^3/3 + ... + s^21/21) leaves out terms
+// below 1e-18 of the result (10 fused multiply-adds; within 2 ulp of log1p
+// over [0, 1]).  1 / ((2 + t or 3 + t) (1 + t)) gives both quotients.
+__device__ __forceinline__ double log1p_recip_unit(double t, double& r1) {
+  const bool hi = t > 0.41421356237309503;
+  const double num = hi ? t - 1.0 : t;
+  const double den = hi ? t + 3.0 : t + 2.0;
+  const double d1 = 1.0 + t;
+  const double inv = recip_small(den * d1);
+  r1 = den * inv;
+  const double s = num * (d1 * inv);
+  const double z = s * s;
+  double p = 1.0 / 21.0;
+#pragma unroll
+  for (int k = 19; k >= 1; k -= 2) p = __builtin_fma(p, z, 1.0 / (double)k);
+  return __builtin_fma(2.0 * s, p, hi ? 0.69314718055994531 : 0.0);
 }
 
 __device__ __forceinline__ double log1p_exp(double a) {

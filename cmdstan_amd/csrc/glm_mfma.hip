@@ -119,8 +119,10 @@ glm_mfma_f64(const double* __restrict__ X, const double* __restrict__ y, int N, 
           // one exp: log1p_exp(e) = max(e,0) + log1p(t), sigmoid(e) from t = e^{-|e|}
           const double yy = sY[rr];
           const double t = exp(-fabs(e));
-          lp += yy * e - ((e > 0.0 ? e : 0.0) + log1p_unit(t));
-          r = (e >= 0.0 ? 1.0 / (1.0 + t) : t / (1.0 + t)) - yy;
+          double r1;
+          const double l1p = log1p_recip_unit(t, r1);
+          lp += yy * e - ((e > 0.0 ? e : 0.0) + l1p);
+          r = (e >= 0.0 ? r1 : t * r1) - yy;
         }
         eta[rb][i] = r;
       }
@@ -262,8 +264,9 @@ glm_mfma_f64_lds(const double* __restrict__ X, const double* __restrict__ y, int
           // t = e^{-|e|}: log1p_exp(e) = max(e,0) + log1p(t), sigmoid(e) from 1 / (1 + t)
           const double yy = sY[b][rr];
           const double t = exp(-fabs(e));
-          const double r1 = recip_1to3(1.0 + t);
-          lp += yy * e - ((e > 0.0 ? e : 0.0) + atanh2_series(t * recip_1to3(2.0 + t)));
+          double r1;
+          const double l1p = log1p_recip_unit(t, r1);
+          lp += yy * e - ((e > 0.0 ? e : 0.0) + l1p);
           r = (e >= 0.0 ? r1 : t * r1) - yy;
         }
         eta[rb][i] = r;

@@ -69,20 +69,6 @@ __device__ __forceinline__ int wave_and(int v) {
 }
 
 // ------------------------------------------------------------ scalar math
-// log(1 + t) for t in [0, 1] as 2 atanh(s), s = t / (2 + t) in [0, 1/3]:
-// 2 (s + s^3/3 + ... + s^31/31).  The first omitted term is below 2e-17 of
-// the result at s = 1/3, so the error is a few ulp; the Horner chain in
-// z = s^2 <= 1/9 is 15 fused multiply-adds after one division (the library
-// log1p evaluates in double-double arithmetic, ~3x the instructions).
-__device__ __forceinline__ double atanh2_series(double s) {
-  const double z = s * s;
-  double p = 1.0 / 31.0;
-#pragma unroll
-  for (int k = 29; k >= 1; k -= 2) p = __builtin_fma(p, z, 1.0 / (double)k);
-  return 2.0 * s * p;
-}
-__device__ __forceinline__ double log1p_unit(double t) { return atanh2_series(t / (2.0 + t)); }
-
 // 1/d for d in [1, 8]: the hardware reciprocal estimate refined by two Newton
 // steps (no range reduction is needed in this range); within an ulp or two of
 // the correctly rounded quotient, at about half the instructions of a division

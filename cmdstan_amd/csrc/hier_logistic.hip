@@ -18,10 +18,13 @@ namespace nutsgpu {
 #define HL_OBS 128   // observations staged per LDS pass
 #define HL_KMAX 16
 
+// log(1 + e^e) and sig = 1 / (1 + e^-e) from one exponential and one reciprocal
 __device__ __forceinline__ double hl_log1p_exp(double e, double* sig) {
   const double t = exp(-fabs(e));
-  *sig = e >= 0.0 ? 1.0 / (1.0 + t) : t / (1.0 + t);
-  return (e > 0.0 ? e : 0.0) + log1p_unit(t);
+  double r1;
+  const double l1p = log1p_recip_unit(t, r1);
+  *sig = e >= 0.0 ? r1 : t * r1;
+  return (e > 0.0 ? e : 0.0) + l1p;
 }
 
 __global__ void __launch_bounds__(256) hl_group_kernel(HlArgs a) {

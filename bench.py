@@ -98,8 +98,8 @@ def make_data(N, D, seed):
 
 def host_threads():
     """Host threads this process may use: the CPU affinity set, capped by
-    OMP_NUM_THREADS when the host sets it (a shared GPU box gives each GPU a
-    CPU share and says so there)."""
+    OMP_NUM_THREADS when the host sets it (a host shared by several GPUs gives
+    each GPU a CPU share and states it there)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     omp = os.environ.get("OMP_NUM_THREADS", "")
     if omp.isdigit() and int(omp) > 0:

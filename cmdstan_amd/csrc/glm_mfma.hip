@@ -316,8 +316,8 @@ __global__ void glm_mfma_reduce(const double* __restrict__ Wg, const double* __r
   double* g = G + (int64_t)c * ldq;
   for (int k = threadIdx.x; k < D; k += blockDim.x) {
     double s = 0.0;
-    // chunk order kept; eight loads in flight instead of one
-#pragma unroll 8
+    // chunk order kept; sixteen loads in flight instead of one
+#pragma unroll 16
     for (int ch = 0; ch < sp.chunks; ++ch) s += Wg[((int64_t)ch * sp.stride + pos) * D + k];
     g[k] = s + s2inv * q[k];
   }
